@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""
+bench.py -- D3Q19 stream-collide throughput of the HIP solver on one MI355X.
+
+Workload: the single-structure precision study lattice, D3Q19 at 256^3 with
+populations stored as shifted FP32 (f - w) and FP64 flux sums, on a seeded
+mask of overlapping 3:1 ellipsoids along a random axis with 15-degree jitter,
+porosity drawn from U(0.3, 0.8).  A "step" is one stream-collide pass over
+the whole lattice (every fluid cell updated once); the lattice stays resident
+in HBM.  value = fluid-cell updates per second (MLUPS) over the timed region.
+
+Printed on one JSON line (stdout): the metric, the roofline of the step
+kernels (algorithmic bytes 2*19*4 = 152 B per fluid update over their average
+duration from HIP events on the solver's stream), the FP64 rate on the same
+lattice, and the reference CPU solver timed on this host on a bounded sample.
+Progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "flow-anizotropy_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_mask(n, seed):
+    from flowperm import masks
+    rng = np.random.default_rng(seed)
+    por = 0.3 + 0.5 * rng.random()
+    t0 = time.time()
+    m = masks.ellipsoids((n, n, n), por, rmin=2.0, rmax=4.0, aspect=3.0, jitter_deg=15.0,
+                         seed=seed)
+    log(f"mask {n}^3 ellipsoids seed {seed}: target porosity {por:.3f}, "
+        f"actual {1 - m.mean():.4f} ({time.time() - t0:.1f} s)")
+    return m, por
+
+
+def bench_steps(ctx, mask, prec, warmup, steps):
+    from flowperm import _lib
+    nz, ny, nx = mask.shape
+    out = _lib.BenchOut()
+    F = np.array([2.5e-7 * 40, 0.0, 0.0])
+    st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, 3, prec,
+                                _lib.f64ptr(F), warmup, steps, C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"fp_bench_steps failed: {st} {ctx.last_error()}")
+    return out
+
+
+def cpu_baseline(mask, budget_s=30.0):
+    """The reference solver (LMB3d/lbm3d-perm.cpp built by oracle/Makefile)
+    on this host, one single-threaded process, on a bounded sample of the same
+    workload: two runs of 1 and 3 steps per direction on the same 256^3 mask;
+    their difference (6 lattice steps) isolates stepping from I/O and setup."""
+    ref = REPO / "oracle" / "_ref" / "lbm3d-perm"
+    from flowperm import structio
+    n_fluid = int(mask.size - np.count_nonzero(mask))
+    with tempfile.TemporaryDirectory() as td:
+        raw = Path(td) / "m.raw"
+        structio.save_raw(raw, mask)
+        if ref.exists():
+            times = {}
+            for n in (1, 3):
+                csv = Path(td) / f"o{n}.csv"
+                t0 = time.perf_counter()
+                p = subprocess.run([str(ref), str(raw), str(csv), "--quiet", "--max-steps", str(n),
+                                    "--avg-steps", "0", "--lag", "1000000", "--ff", "40"],
+                                   capture_output=True, text=True, timeout=600)
+                times[n] = time.perf_counter() - t0
+                if p.returncode not in (0, 3):
+                    raise RuntimeError(f"reference failed: {p.stderr[-300:]}")
+            dt = times[3] - times[1]
+            updates = 3 * 2 * n_fluid
+            return {"value": updates / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "reference",
+                    "sample": f"lbm3d-perm built from the reference source (-O3 -march=broadwell "
+                              f"-ffast-math) on the same 256^3 mask: runs of 3x1 and 3x3 steps "
+                              f"(--max-steps 1 / 3 --avg-steps 0), difference = 6 lattice steps "
+                              f"= {updates} fluid updates in {dt:.2f} s "
+                              f"(wall {times[1]:.1f} s + {times[3]:.1f} s)"}
+        sys.path.insert(0, str(REPO))
+        from oracle import binding
+        t0 = time.perf_counter()
+        binding.run_steps3d(mask, (1e-5, 0, 0), 1)
+        t1 = time.perf_counter()
+        binding.run_steps3d(mask, (1e-5, 0, 0), 3)
+        t2 = time.perf_counter()
+        dt = (t2 - t1) - (t1 - t0)
+        return {"value": 2 * n_fluid / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+                "sample": "CPU oracle (oracle/lbm3d_oracle.c), 3 vs 1 steps on the same mask"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this solver has no multi-GPU path (batches shard by "
+              "instance with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from flowperm import solver
+    mask, por_target = workload_mask(a.size, a.seed)
+    n_fluid = int(mask.size - np.count_nonzero(mask))
+    ctx = solver.Context()
+    steps = max(2, a.steps & ~1)
+    warm = max(0, a.warmup & ~1)
+
+    log(f"FP32 D3Q19 {a.size}^3: warmup {warm}, timed {steps} steps")
+    o32 = bench_steps(ctx, mask, 1, warm, steps)
+    ms_step = o32.total_ms / o32.steps
+    mlups = o32.n_fluid * o32.steps / (o32.total_ms / 1e3) / 1e6
+    kern_ms = 0.5 * (o32.local_ms + o32.nbr_ms)
+    bytes_launch = o32.n_fluid * o32.bytes_per_update
+    achieved = bytes_launch / (kern_ms / 1e3) / 1e9
+    log(f"FP32: {mlups:.1f} MLUPS, {ms_step:.4f} ms/step; local {o32.local_ms:.4f} ms, "
+        f"neighbour {o32.nbr_ms:.4f} ms, monitor {o32.monitor_ms:.4f} ms; "
+        f"step kernels {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.3f} of peak")
+
+    extra = {}
+    if not a.no_fp64:
+        s64 = max(2, (steps // 2) & ~1)
+        o64 = bench_steps(ctx, mask, 0, max(2, warm // 2), s64)
+        k64 = 0.5 * (o64.local_ms + o64.nbr_ms)
+        extra["fp64"] = {
+            "mlups": o64.n_fluid * o64.steps / (o64.total_ms / 1e3) / 1e6,
+            "ms_per_step": o64.total_ms / o64.steps,
+            "local_ms": o64.local_ms, "neighbour_ms": o64.nbr_ms,
+            "step_kernel_GBs": o64.n_fluid * o64.bytes_per_update / (k64 / 1e3) / 1e9}
+        log(f"FP64: {extra['fp64']['mlups']:.1f} MLUPS, {extra['fp64']['step_kernel_GBs']:.0f} GB/s")
+    ctx.close()
+
+    cpu = None
+    if not a.no_cpu:
+        log("CPU baseline: reference solver on this host ...")
+        try:
+            cpu = cpu_baseline(mask)
+            log(f"CPU: {cpu['value']:.2f} MLUPS ({cpu['kind']})")
+        except Exception as e:                       # reported, never fatal to the GPU number
+            log(f"CPU baseline failed: {e}")
+
+    line = {
+        "metric": "D3Q19 MLUPS + %HBM peak at 256^3; batched K-tensor solves/s, p99 latency",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": int(o32.steps),
+        "warmup": warm,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"D3Q19 stream-collide, FP32 shifted populations (FP64 flux sums), "
+                        f"{a.size}^3 periodic lattice of overlapping 3:1 ellipsoids "
+                        f"(15 deg jitter, seed {a.seed}), porosity {1 - mask.mean():.4f}",
+            "lattice": [a.size] * 3, "n_fluid": n_fluid, "n_cells": int(mask.size),
+            "force": "2.5e-7 * 40 along x", "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_local/k_nbr<D3Q19,F32>: 152 B per fluid update x n_fluid per launch "
+                      "over the mean event-timed duration of the two step kernels",
+            "local_ms": round(o32.local_ms, 5), "neighbour_ms": round(o32.nbr_ms, 5),
+            "monitor_ms": round(o32.monitor_ms, 5)},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

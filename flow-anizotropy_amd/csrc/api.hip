@@ -1,0 +1,814 @@
+// api.hip -- the extern "C" surface of libflowperm.so (include/flowperm.h).
+#include "../../include/flowperm.h"
+#include "engine.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace fp;
+
+struct fp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+const double FX_BASE = 2.5e-07;               // LMB3d/lbm3d-perm.cpp:115, LMB2d/main.cpp:62
+const double TAU = 1.0, CS2 = 1.0 / 3.0;
+const double MU3 = CS2 * (TAU - 0.5);         // LMB3d/lbm3d-perm.cpp:114
+
+float mu2d()                                  // LMB2d/lbm.cpp:44-45
+{
+    float tau = 1;
+    float mu = 0.33 * (tau - 0.5);
+    return mu;
+}
+
+int fail(fp_ctx *ctx, int status, const std::string &msg)
+{
+    if(ctx) ctx->err = msg;
+    return status;
+}
+
+int status_of_geometry_error(const std::string &e)
+{
+    if(e == "bad dimensions") return FP_ERR_BAD_DIMS;
+    if(e == "non-binary voxel") return FP_ERR_NON_BINARY;
+    if(e == "structure has no pore space") return FP_ERR_NO_PORE;
+    if(e.find("hipMalloc") != std::string::npos) return FP_ERR_OOM;
+    if(e.find("too many") != std::string::npos || e.find("too large") != std::string::npos)
+        return FP_ERR_ARG;
+    return FP_ERR_DEVICE;
+}
+
+InstCfg cfg3(const fp_config3d &c, int64_t nn, bool want_vel)
+{
+    InstCfg k;
+    std::memset(&k, 0, sizeof(k));
+    k.eps = c.eps;
+    k.lag = c.lag;
+    k.min_steps = c.min_steps;
+    k.max_steps = c.max_steps;
+    k.avg_steps = c.avg_steps;
+    k.avg_sample = 1;
+    k.hold = c.hold;
+    k.kind = 3;
+    k.want_vel = want_vel ? 1 : 0;
+    k.denom[0] = k.denom[1] = k.denom[2] = (double)nn;   // q = sum / NN  (:302)
+    return k;
+}
+
+InstCfg cfg2(const fp_config2d &c, int lx, int ly)
+{
+    InstCfg k;
+    std::memset(&k, 0, sizeof(k));
+    k.eps = c.eps;
+    k.qfloor = c.qfloor;
+    k.ksigma = c.ksigma;
+    k.lag = c.lag;
+    k.min_steps = c.min_steps;
+    k.max_steps = c.max_steps;
+    k.avg_steps = c.avg_steps;
+    k.avg_sample = c.avg_sample;
+    k.hold = c.hold;
+    k.kind = 2;
+    k.want_vel = 1;
+    // q_x = su / (LX * L0 * L0),  q_y = sv / (LY * L0 * L0)   (LMB2d/lbm.cpp:150-151)
+    k.denom[0] = (double)lx * c.L0 * c.L0;
+    k.denom[1] = (double)ly * c.L0 * c.L0;
+    k.denom[2] = 1.0;
+    return k;
+}
+
+// compact field (dim floats per fluid cell, .raw order) -> dense, zero at solid
+void scatter_field(const Geometry &g, const std::vector<float> &compact, float *dense)
+{
+    const int dim = g.dim;
+    int64_t c = 0;
+    for(int64_t n = 0; n < g.nn; n++) {
+        if(g.solid[n] == 0) {
+            for(int k = 0; k < dim; k++) dense[n * dim + k] = compact[c * dim + k];
+            c++;
+        } else {
+            for(int k = 0; k < dim; k++) dense[n * dim + k] = 0.0f;
+        }
+    }
+}
+
+std::string copy_field(const Job &j, int slot, std::vector<float> &out)
+{
+    out.resize((size_t)j.geo->dim * j.geo->n_fluid);
+    hipError_t e = hipMemcpy(out.data(), j.d_vel[slot], out.size() * sizeof(float), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? "" : std::string("hipMemcpy(field): ") + hipGetErrorString(e);
+}
+
+// field_residual of LMB2d/lbm.cpp:169-182 over dense [y][x] pairs, i-outer order
+double field_residual(int lx, int ly, const float *cur, const float *snap)
+{
+    double num = 0.0, den = 0.0;
+    for(int i = 0; i < lx; i++)
+        for(int j = 0; j < ly; j++) {
+            const size_t n = ((size_t)j * lx + i) * 2;
+            double du = cur[n] - snap[n];
+            double dv = cur[n + 1] - snap[n + 1];
+            num += du * du + dv * dv;
+            den += (double)cur[n] * cur[n] + (double)cur[n + 1] * cur[n + 1];
+        }
+    return den > 0.0 ? (double)(float)std::sqrt(num / den) : 10.0;
+}
+
+void fill_out(const Job &j, fp_solve_out *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    for(int k = 0; k < 3; k++) { o->q[k] = j.state.q[k]; o->se[k] = j.state.se[k]; }
+    o->steps = j.state.steps;
+    o->converged = j.state.converged;
+    o->status = j.state.status == 2 ? FP_ERR_NONFINITE : FP_OK;
+}
+
+// Run a structure's forced solves to completion: together on one batch, or one
+// after another.  The jobs keep their device buffers (the caller reads the
+// velocity fields, then frees them with free_job_memory).
+std::string run_jobs(fp_ctx *ctx, std::vector<Job *> &jobs, int lattice, int prec, bool together)
+{
+    int64_t maxsteps = 0;
+    for(Job *j : jobs)
+        maxsteps = std::max<int64_t>(maxsteps, std::max<int64_t>(0, j->cfg.max_steps) +
+                                                   std::max<int64_t>(0, j->cfg.avg_steps));
+    const int chunk = (int)std::min<int64_t>(200, std::max<int64_t>(2, (maxsteps + 1) & ~1ll));
+    auto step_until_done = [&](const std::vector<Job *> &set) -> std::string {
+        Batch b(lattice, prec, ctx->stream, chunk, true);
+        std::string err;
+        for(Job *j : set) {
+            err = b.admit(j);
+            if(!err.empty()) break;
+        }
+        while(err.empty()) {
+            bool all = true;
+            for(Job *j : set) all = all && j->done;
+            if(all) break;
+            err = b.run_chunk();
+            (void)b.take_done();
+        }
+        b.detach_all();
+        return err;
+    };
+    std::string err;
+    if(together) err = step_until_done(jobs);
+    else
+        for(Job *j : jobs) {
+            err = step_until_done(std::vector<Job *>{j});
+            if(!err.empty()) break;
+        }
+    return err;
+}
+
+void free_job_memory(Job &j)
+{
+    if(j.d_f) { (void)hipFree(j.d_f); j.d_f = nullptr; }
+    for(int s = 0; s < 2; s++)
+        if(j.d_vel[s]) { (void)hipFree(j.d_vel[s]); j.d_vel[s] = nullptr; }
+}
+
+// symmetric 3x3 cyclic Jacobi, LMB3d/lbm3d-perm.cpp:401-443
+void jacobi3(const double Ain[3][3], double val[3], double vec[3][3])
+{
+    double a[3][3];
+    std::memcpy(a, Ain, sizeof(a));
+    for(int i = 0; i < 3; i++) for(int j = 0; j < 3; j++) vec[i][j] = (i == j) ? 1.0 : 0.0;
+    for(int sweep = 0; sweep < 50; sweep++) {
+        double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
+        if(off < 1e-18) break;
+        for(int p = 0; p < 2; p++)
+            for(int q = p + 1; q < 3; q++) {
+                if(std::fabs(a[p][q]) < 1e-300) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) /
+                                 (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for(int k = 0; k < 3; k++) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;  a[k][q] = s * akp + c * akq;
+                }
+                for(int k = 0; k < 3; k++) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;  a[q][k] = s * apk + c * aqk;
+                    const double vkp = vec[k][p], vkq = vec[k][q];
+                    vec[k][p] = c * vkp - s * vkq; vec[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for(int i = 0; i < 3; i++) val[i] = a[i][i];
+    for(int i = 0; i < 2; i++)
+        for(int j = i + 1; j < 3; j++)
+            if(val[j] > val[i]) {
+                std::swap(val[i], val[j]);
+                for(int k = 0; k < 3; k++) std::swap(vec[k][i], vec[k][j]);
+            }
+}
+
+int check_mask(const uint8_t *solid, int64_t nn, int64_t *n_pore)
+{
+    int64_t np = 0;
+    for(int64_t i = 0; i < nn; i++) {
+        if(solid[i] > 1) return FP_ERR_NON_BINARY;
+        if(!solid[i]) np++;
+    }
+    if(n_pore) *n_pore = np;
+    return np == 0 ? FP_ERR_NO_PORE : FP_OK;
+}
+
+const char *msg_of(int st)
+{
+    switch(st) {
+    case FP_ERR_BAD_DIMS: return "bad dimensions";
+    case FP_ERR_NON_BINARY: return "non-binary voxel";
+    case FP_ERR_NO_PORE: return "structure has no pore space";
+    default: return "error";
+    }
+}
+
+bool cfg3_ok(const fp_config3d *c) { return c && c->lag > 0; }
+bool cfg2_ok(const fp_config2d *c) { return c && c->lag > 0 && c->avg_sample > 0 && c->L0 > 0; }
+
+} // namespace
+
+// ===========================================================================
+extern "C" {
+
+int fp_exit_code(int status)
+{
+    if(status == FP_OK) return 0;
+    if(status == FP_NOT_CONVERGED) return 3;
+    return 2;
+}
+
+int fp_abi_version(void) { return FP_ABI_VERSION; }
+
+fp_ctx *fp_create(int device, int *status)
+{
+    fp_ctx *c = new(std::nothrow) fp_ctx;
+    if(!c) { if(status) *status = FP_ERR_OOM; return nullptr; }
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if(e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if(e != hipSuccess) {
+        if(status) *status = FP_ERR_DEVICE;
+        delete c;
+        return nullptr;
+    }
+    if(status) *status = FP_OK;
+    return c;
+}
+
+void fp_destroy(fp_ctx *ctx)
+{
+    if(!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if(ctx->stream) { (void)hipStreamSynchronize(ctx->stream); (void)hipStreamDestroy(ctx->stream); }
+    delete ctx;
+}
+
+const char *fp_last_error(fp_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+void fp_config3d_default(fp_config3d *c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->ff = 1.0; c->eps = 1e-5; c->lag = 2000; c->hold = 2;
+    c->min_steps = 4000; c->max_steps = 200000; c->avg_steps = 2000;
+    c->validate = 0; c->precision = FP_F64; c->concurrent = 0;
+}
+
+void fp_config2d_default(fp_config2d *c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->ff = 1.0; c->eps = 1e-3; c->qfloor = 1e-2; c->ksigma = 3.0; c->L0 = 4.0;
+    c->validate_deg = -1.0;
+    c->lag = 5000; c->min_steps = 10000; c->max_steps = 500000; c->avg_steps = 20000;
+    c->avg_sample = 50; c->hold = 3; c->precision = FP_F32_REF2D; c->concurrent = 0;
+}
+
+void fp_jacobi3(const double A9[9], double val[3], double vec9[9])
+{
+    double A[3][3], vec[3][3];
+    std::memcpy(A, A9, sizeof(A));
+    jacobi3(A, val, vec);
+    std::memcpy(vec9, vec, sizeof(vec));
+}
+
+int fp_check_mask3d(const uint8_t *solid, int nx, int ny, int nz, int64_t *n_pore)
+{
+    if(nx <= 0 || ny <= 0 || nz <= 0) return FP_ERR_BAD_DIMS;
+    if(!solid) return FP_ERR_ARG;
+    return check_mask(solid, (int64_t)nx * ny * nz, n_pore);
+}
+
+// ---------------------------------------------------------------------------
+// 3D
+// ---------------------------------------------------------------------------
+static int solve3d_dirs(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
+                        const fp_config3d *cfg, const double (*F)[3], int ndir,
+                        fp_solve_out *outs, float *vel, int64_t *updates)
+{
+    if(!ctx) return FP_ERR_ARG;
+    if(!cfg3_ok(cfg)) return fail(ctx, FP_ERR_ARG, "lag must be positive");
+    if(cfg->precision != FP_F64 && cfg->precision != FP_F32)
+        return fail(ctx, FP_ERR_ARG, "3D precision must be FP_F64 or FP_F32");
+    if(nx <= 0 || ny <= 0 || nz <= 0) return fail(ctx, FP_ERR_BAD_DIMS, "bad dimensions");
+    int64_t np = 0;
+    int st = check_mask(solid, (int64_t)nx * ny * nz, &np);
+    if(st != FP_OK) return fail(ctx, st, msg_of(st));
+    (void)hipSetDevice(ctx->device);
+    Geometry g;
+    std::string e = geometry_build(g, solid, 3, nx, ny, nz);
+    if(!e.empty()) return fail(ctx, status_of_geometry_error(e), e);
+    std::vector<Job> jobs(ndir);
+    std::vector<Job *> ptrs;
+    for(int d = 0; d < ndir; d++) {
+        Job &j = jobs[d];
+        j.geo = &g;
+        j.lattice = 3;
+        j.prec = cfg->precision;
+        for(int k = 0; k < 3; k++) j.F[k] = F[d][k];
+        j.cfg = cfg3(*cfg, g.nn, vel != nullptr);
+        j.want_fields = vel != nullptr;
+        ptrs.push_back(&j);
+    }
+    e = run_jobs(ctx, ptrs, 3, cfg->precision, cfg->concurrent != 0);
+    int rc = FP_OK;
+    if(!e.empty()) rc = fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
+    int64_t upd = 0;
+    for(int d = 0; d < ndir && rc == FP_OK; d++) {
+        fill_out(jobs[d], &outs[d]);
+        upd += jobs[d].state.steps * g.n_fluid;
+        if(vel) {
+            std::vector<float> c;
+            std::string ee = copy_field(jobs[d], 1, c);
+            if(!ee.empty()) { rc = fail(ctx, FP_ERR_DEVICE, ee); break; }
+            scatter_field(g, c, vel + (size_t)d * g.nn * 3);
+        }
+    }
+    for(Job &j : jobs) free_job_memory(j);
+    if(updates) *updates = upd;
+    return rc;
+}
+
+int fp_solve_dir3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
+                   const fp_config3d *cfg, const double F[3], fp_solve_out *out, float *vel)
+{
+    const double Fd[1][3] = {{F[0], F[1], F[2]}};
+    int rc = solve3d_dirs(ctx, solid, nx, ny, nz, cfg, Fd, 1, out, vel, nullptr);
+    if(rc != FP_OK) return rc;
+    if(out->status != FP_OK)
+        return fail(ctx, FP_ERR_NONFINITE, "non-finite flux in run at step " +
+                                            std::to_string(out->steps));
+    return out->converged ? FP_OK : FP_NOT_CONVERGED;
+}
+
+int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
+               const fp_config3d *cfg, fp_result3d *out, float *vel)
+{
+    if(!ctx || !out || !cfg) return FP_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    const double t0 = now_seconds();
+    const double F = FX_BASE * cfg->ff;
+    const double Fd[3][3] = {{F, 0, 0}, {0, F, 0}, {0, 0, F}};
+    int64_t upd = 0;
+    int rc = solve3d_dirs(ctx, solid, nx, ny, nz, cfg, Fd, 3, out->run, vel, &upd);
+    if(rc != FP_OK) return rc;
+    const char *tags[3] = {"fx", "fy", "fz"};
+    for(int d = 0; d < 3; d++)
+        if(out->run[d].status != FP_OK)
+            return fail(ctx, FP_ERR_NONFINITE, std::string("non-finite flux in run ") + tags[d] +
+                                                   " at step " + std::to_string(out->run[d].steps));
+    int64_t np = 0;
+    (void)check_mask(solid, (int64_t)nx * ny * nz, &np);
+    out->nx = nx; out->ny = ny; out->nz = nz;
+    out->n_pore = np;
+    const double NN = (double)((int64_t)nx * ny * nz);
+    out->porosity = (double)np / NN;
+    out->force = F;
+    out->mu = MU3;
+
+    // LMB3d/lbm3d-perm.cpp:523-553
+    double (*K)[3] = out->K;
+    for(int c = 0; c < 3; c++) {
+        K[c][0] = MU3 * out->run[0].q[c] / F;
+        K[c][1] = MU3 * out->run[1].q[c] / F;
+        K[c][2] = MU3 * out->run[2].q[c] / F;
+    }
+    const double trace = K[0][0] + K[1][1] + K[2][2];
+    out->k_xy = 0.5 * (K[0][1] + K[1][0]);
+    out->k_xz = 0.5 * (K[0][2] + K[2][0]);
+    out->k_yz = 0.5 * (K[1][2] + K[2][1]);
+    out->recip_xy = std::fabs(K[0][1] - K[1][0]) / std::fabs(trace);
+    out->recip_xz = std::fabs(K[0][2] - K[2][0]) / std::fabs(trace);
+    out->recip_yz = std::fabs(K[1][2] - K[2][1]) / std::fabs(trace);
+    out->recip_resid = dmax_ref(out->recip_xy, dmax_ref(out->recip_xz, out->recip_yz));
+    double Ks[3][3] = {{K[0][0], out->k_xy, out->k_xz},
+                       {out->k_xy, K[1][1], out->k_yz},
+                       {out->k_xz, out->k_yz, K[2][2]}};
+    double *val = out->k;
+    jacobi3(Ks, val, out->v);
+    out->k_mean = trace / 3.0;
+    const double dv = (val[0]-val[1])*(val[0]-val[1]) + (val[1]-val[2])*(val[1]-val[2])
+                    + (val[2]-val[0])*(val[2]-val[0]);
+    const double nrm = val[0]*val[0] + val[1]*val[1] + val[2]*val[2];
+    out->fa = nrm > 0 ? std::sqrt(0.5 * dv / nrm) : NAN;
+
+    if(cfg->validate) {                                  // :555-572
+        const double c = 1.0 / std::sqrt(3.0);
+        const double Fv[1][3] = {{F * c, F * c, F * c}};
+        int64_t u2 = 0;
+        rc = solve3d_dirs(ctx, solid, nx, ny, nz, cfg, Fv, 1, &out->val, nullptr, &u2);
+        if(rc != FP_OK) return rc;
+        if(out->val.status != FP_OK)
+            return fail(ctx, FP_ERR_NONFINITE, "non-finite flux in run val at step " +
+                                                   std::to_string(out->val.steps));
+        upd += u2;
+        double err = 0, qn = 0;
+        for(int r = 0; r < 3; r++) {
+            out->val_pred[r] = (Ks[r][0]*F*c + Ks[r][1]*F*c + Ks[r][2]*F*c) / MU3;
+            err += (out->val.q[r] - out->val_pred[r]) * (out->val.q[r] - out->val_pred[r]);
+            qn += out->val.q[r] * out->val.q[r];
+        }
+        out->val_err = std::sqrt(err) / (qn > 0 ? std::sqrt(qn) : 1);
+        out->validated = 1;
+    }
+    out->seconds = now_seconds() - t0;
+    out->fluid_updates = upd;
+    ctx->err.clear();
+    return (out->run[0].converged && out->run[1].converged && out->run[2].converged)
+               ? FP_OK : FP_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------
+// 2D
+// ---------------------------------------------------------------------------
+static int solve2d_dirs(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const fp_config2d *cfg,
+                        const double (*F)[2], int ndir, fp_solve_out *outs, float *vel, int64_t *updates)
+{
+    if(!ctx) return FP_ERR_ARG;
+    if(!cfg2_ok(cfg)) return fail(ctx, FP_ERR_ARG, "--lag and --avg-sample must be positive");
+    if(cfg->precision != FP_F64 && cfg->precision != FP_F32 && cfg->precision != FP_F32_REF2D)
+        return fail(ctx, FP_ERR_ARG, "unknown precision");
+    if(lx <= 0 || ly <= 0) return fail(ctx, FP_ERR_BAD_DIMS, "bad dimensions");
+    int64_t np = 0;
+    int st = check_mask(solid, (int64_t)lx * ly, &np);
+    if(st == FP_ERR_NON_BINARY) return fail(ctx, st, msg_of(st));
+    if(st == FP_ERR_NO_PORE) return fail(ctx, st, msg_of(st));
+    (void)hipSetDevice(ctx->device);
+    Geometry g;
+    std::string e = geometry_build(g, solid, 2, lx, ly, 1);
+    if(!e.empty()) return fail(ctx, status_of_geometry_error(e), e);
+    std::vector<Job> jobs(ndir);
+    std::vector<Job *> ptrs;
+    for(int d = 0; d < ndir; d++) {
+        Job &j = jobs[d];
+        j.geo = &g;
+        j.lattice = 2;
+        j.prec = cfg->precision;
+        j.F[0] = F[d][0]; j.F[1] = F[d][1]; j.F[2] = 0.0;
+        j.cfg = cfg2(*cfg, lx, ly);
+        j.want_fields = true;
+        ptrs.push_back(&j);
+    }
+    e = run_jobs(ctx, ptrs, 2, cfg->precision, cfg->concurrent != 0);
+    int rc = FP_OK;
+    if(!e.empty()) rc = fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
+    int64_t upd = 0;
+    const size_t nn = (size_t)lx * ly;
+    std::vector<float> snap(nn * 2), cur(nn * 2), c;
+    for(int d = 0; d < ndir && rc == FP_OK; d++) {
+        fill_out(jobs[d], &outs[d]);
+        upd += jobs[d].state.steps * g.n_fluid;
+        if(jobs[d].state.status != 0) continue;
+        std::string ee = copy_field(jobs[d], 0, c);
+        if(ee.empty()) scatter_field(g, c, snap.data());
+        if(ee.empty() && cfg->avg_steps > 0) {
+            ee = copy_field(jobs[d], 1, c);
+            if(ee.empty()) scatter_field(g, c, cur.data());
+        } else cur = snap;
+        if(!ee.empty()) { rc = fail(ctx, FP_ERR_DEVICE, ee); break; }
+        outs[d].fieldres = field_residual(lx, ly, cur.data(), snap.data());
+        if(vel) std::memcpy(vel + (size_t)d * nn * 2, cur.data(), nn * 2 * sizeof(float));
+    }
+    for(Job &j : jobs) free_job_memory(j);
+    if(updates) *updates = upd;
+    return rc;
+}
+
+int fp_solve_dir2d(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const fp_config2d *cfg,
+                   const double F[2], fp_solve_out *out, float *vel)
+{
+    const double Fd[1][2] = {{F[0], F[1]}};
+    int rc = solve2d_dirs(ctx, solid, lx, ly, cfg, Fd, 1, out, vel, nullptr);
+    if(rc != FP_OK) return rc;
+    if(out->status != FP_OK)
+        return fail(ctx, FP_ERR_NONFINITE, "non-finite flux in run at step " + std::to_string(out->steps));
+    return out->converged ? FP_OK : FP_NOT_CONVERGED;
+}
+
+int fp_solve2d(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const fp_config2d *cfg,
+               fp_result2d *out, float *vel)
+{
+    if(!ctx || !out || !cfg) return FP_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    const double t0 = now_seconds();
+    const double F = FX_BASE * cfg->ff;                 // main.cpp:314
+    const double Fd[2][2] = {{F, 0.0}, {0.0, F}};
+    int64_t upd = 0;
+    int rc = solve2d_dirs(ctx, solid, lx, ly, cfg, Fd, 2, out->run, vel, &upd);
+    if(rc != FP_OK) return rc;
+    const char *tags[2] = {"fx", "fy"};
+    for(int d = 0; d < 2; d++)
+        if(out->run[d].status != FP_OK)
+            return fail(ctx, FP_ERR_NONFINITE, std::string("non-finite flux in run ") + tags[d] +
+                                                   " at step " + std::to_string(out->run[d].steps));
+    int nfluid = 0;                                      // getporosity_full, lbm.cpp:188-195
+    for(int64_t i = 0; i < (int64_t)lx * ly; i++) if(solid[i] == 0) nfluid++;
+    const float por = nfluid / (float)(lx * ly);
+    out->lx = lx; out->ly = ly;
+    out->porosity = por;
+    out->force = F;
+    const float mu = mu2d();
+    out->mu = mu;
+    const fp_solve_out &rx = out->run[0], &ry = out->run[1];
+    // main.cpp:327-353
+    const double k_xx = mu * rx.q[0] / F;
+    const double k_yx = mu * rx.q[1] / F;
+    const double k_xy = mu * ry.q[0] / F;
+    const double k_yy = mu * ry.q[1] / F;
+    const double k_off = 0.5 * (k_xy + k_yx);
+    const double trace = k_xx + k_yy;
+    const double recip = trace != 0.0 ? std::fabs(k_xy - k_yx) / std::fabs(trace) : NAN;
+    const double km = 0.5 * (k_xx + k_yy);
+    const double d1 = 0.5 * (k_xx - k_yy);
+    const double d2 = k_off;
+    const double r = std::sqrt(d1 * d1 + d2 * d2);
+    out->k_xx = k_xx; out->k_yx = k_yx; out->k_xy = k_xy; out->k_yy = k_yy;
+    out->k_off = k_off; out->recip_resid = recip;
+    out->k1 = km + r; out->k2 = km - r; out->k_mean = km;
+    out->anisotropy = km != 0.0 ? r / km : NAN;
+    out->theta_deg = 0.5 * std::atan2(d2, d1) * 180.0 / M_PI;
+    out->alpha_x_deg = std::atan2(rx.q[1], rx.q[0]) * 180.0 / M_PI;
+    out->alpha_y_deg = std::atan2(ry.q[1], ry.q[0]) * 180.0 / M_PI - 90.0;
+
+    if(cfg->validate_deg >= 0.0) {                       // main.cpp:360-380
+        const double th = cfg->validate_deg * M_PI / 180.0;
+        const double Fx = F * std::cos(th), Fy = F * std::sin(th);
+        const double Fv[1][2] = {{Fx, Fy}};
+        int64_t u2 = 0;
+        rc = solve2d_dirs(ctx, solid, lx, ly, cfg, Fv, 1, &out->val, nullptr, &u2);
+        if(rc != FP_OK) return rc;
+        if(out->val.status != FP_OK)
+            return fail(ctx, FP_ERR_NONFINITE, "non-finite flux while averaging run val");
+        upd += u2;
+        const double pred_qx = (k_xx * Fx + k_off * Fy) / mu;
+        const double pred_qy = (k_off * Fx + k_yy * Fy) / mu;
+        const double qn = std::sqrt(out->val.q[0] * out->val.q[0] + out->val.q[1] * out->val.q[1]);
+        out->val_pred[0] = pred_qx; out->val_pred[1] = pred_qy;
+        out->val_err = std::sqrt((out->val.q[0] - pred_qx) * (out->val.q[0] - pred_qx) +
+                                 (out->val.q[1] - pred_qy) * (out->val.q[1] - pred_qy)) / (qn > 0 ? qn : 1);
+        out->validated = 1;
+    }
+    out->seconds = now_seconds() - t0;
+    out->fluid_updates = upd;
+    ctx->err.clear();
+    return (rx.converged && ry.converged) ? FP_OK : FP_NOT_CONVERGED;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+struct fp_batch {
+    fp_ctx *ctx = nullptr;
+    int lattice = 3, prec = FP_F64;
+    std::unique_ptr<Batch> b;
+    std::vector<std::unique_ptr<Geometry>> geos;
+    std::unordered_multimap<uint64_t, Geometry *> geo_index;
+    std::vector<std::unique_ptr<Job>> jobs;          // index = job id
+    std::vector<fp_solve_out> results;
+    std::vector<char> have;
+    int64_t updates = 0;
+    double seconds = 0;
+};
+
+static uint64_t fnv1a(const uint8_t *p, size_t n, uint64_t h)
+{
+    for(size_t i = 0; i < n; i++) { h ^= p[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+// one device geometry per distinct mask
+static Geometry *batch_geometry(fp_batch *B, const uint8_t *solid, int dim, int nx, int ny, int nz,
+                                std::string &err)
+{
+    const int64_t nn = (int64_t)nx * ny * (dim == 3 ? nz : 1);
+    uint64_t h = fnv1a(solid, (size_t)nn, 1469598103934665603ull);
+    h = fnv1a((const uint8_t *)&nx, sizeof(int), h);
+    h = fnv1a((const uint8_t *)&ny, sizeof(int), h);
+    h = fnv1a((const uint8_t *)&nz, sizeof(int), h);
+    auto range = B->geo_index.equal_range(h);
+    for(auto it = range.first; it != range.second; ++it) {
+        Geometry *g = it->second;
+        if(g->dim == dim && g->nx == nx && g->ny == ny && g->nz == (dim == 3 ? nz : 1) &&
+           std::memcmp(g->solid.data(), solid, (size_t)nn) == 0)
+            return g;
+    }
+    std::unique_ptr<Geometry> g(new Geometry);
+    err = geometry_build(*g, solid, dim, nx, ny, nz);
+    if(!err.empty()) return nullptr;
+    Geometry *raw = g.get();
+    B->geos.push_back(std::move(g));
+    B->geo_index.emplace(h, raw);
+    return raw;
+}
+
+extern "C" {
+
+fp_batch *fp_batch_create(fp_ctx *ctx, int lattice, int precision, int *status)
+{
+    if(!ctx || (lattice != 2 && lattice != 3) ||
+       (lattice == 3 && precision != FP_F64 && precision != FP_F32) ||
+       (lattice == 2 && precision != FP_F64 && precision != FP_F32 && precision != FP_F32_REF2D)) {
+        if(status) *status = FP_ERR_ARG;
+        return nullptr;
+    }
+    fp_batch *B = new(std::nothrow) fp_batch;
+    if(!B) { if(status) *status = FP_ERR_OOM; return nullptr; }
+    B->ctx = ctx;
+    B->lattice = lattice;
+    B->prec = precision;
+    B->b.reset(new Batch(lattice, precision, ctx->stream, 200, true));
+    if(status) *status = FP_OK;
+    return B;
+}
+
+void fp_batch_destroy(fp_batch *B)
+{
+    if(!B) return;
+    B->b.reset();
+    delete B;
+}
+
+static int64_t batch_add(fp_batch *B, Geometry *g, const InstCfg &cfg, const double F[3])
+{
+    std::unique_ptr<Job> j(new Job);
+    j->geo = g;
+    j->lattice = B->lattice;
+    j->prec = B->prec;
+    for(int k = 0; k < 3; k++) j->F[k] = F[k];
+    j->cfg = cfg;
+    j->want_fields = (B->lattice == 2);
+    j->id = (int64_t)B->jobs.size();
+    std::string e = B->b->admit(j.get());
+    if(!e.empty()) {
+        fail(B->ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
+        return e.find("memory") != std::string::npos ? -FP_ERR_OOM : -FP_ERR_DEVICE;
+    }
+    B->jobs.push_back(std::move(j));
+    B->results.emplace_back();
+    B->have.push_back(0);
+    return (int64_t)B->jobs.size() - 1;
+}
+
+int64_t fp_batch_add3d(fp_batch *B, const uint8_t *solid, int nx, int ny, int nz,
+                       const fp_config3d *cfg, const double F[3])
+{
+    if(!B || !solid || !cfg3_ok(cfg) || B->lattice != 3) return -FP_ERR_ARG;
+    if(nx <= 0 || ny <= 0 || nz <= 0) return -FP_ERR_BAD_DIMS;
+    int st = check_mask(solid, (int64_t)nx * ny * nz, nullptr);
+    if(st != FP_OK) return -st;
+    (void)hipSetDevice(B->ctx->device);
+    std::string err;
+    Geometry *g = batch_geometry(B, solid, 3, nx, ny, nz, err);
+    if(!g) return -(int64_t)status_of_geometry_error(err);
+    return batch_add(B, g, cfg3(*cfg, g->nn, false), F);
+}
+
+int64_t fp_batch_add2d(fp_batch *B, const uint8_t *solid, int lx, int ly,
+                       const fp_config2d *cfg, const double F[2])
+{
+    if(!B || !solid || !cfg2_ok(cfg) || B->lattice != 2) return -FP_ERR_ARG;
+    if(lx <= 0 || ly <= 0) return -FP_ERR_BAD_DIMS;
+    int st = check_mask(solid, (int64_t)lx * ly, nullptr);
+    if(st != FP_OK) return -st;
+    (void)hipSetDevice(B->ctx->device);
+    std::string err;
+    Geometry *g = batch_geometry(B, solid, 2, lx, ly, 1, err);
+    if(!g) return -(int64_t)status_of_geometry_error(err);
+    const double F3[3] = {F[0], F[1], 0.0};
+    return batch_add(B, g, cfg2(*cfg, lx, ly), F3);
+}
+
+int fp_batch_run(fp_batch *B)
+{
+    if(!B) return FP_ERR_ARG;
+    (void)hipSetDevice(B->ctx->device);
+    const double t0 = now_seconds();
+    auto collect = [&]() {
+        for(Job *j : B->b->take_done()) {
+            fill_out(*j, &B->results[(size_t)j->id]);
+            B->have[(size_t)j->id] = 1;
+            B->updates += j->state.steps * j->geo->n_fluid;
+            B->b->release(j);
+        }
+    };
+    collect();
+    while(B->b->live() > 0) {
+        std::string e = B->b->run_chunk();
+        if(!e.empty()) return fail(B->ctx, FP_ERR_DEVICE, e);
+        collect();
+    }
+    B->seconds += now_seconds() - t0;
+    return FP_OK;
+}
+
+int fp_batch_result(fp_batch *B, int64_t id, fp_solve_out *out)
+{
+    if(!B || !out) return FP_ERR_ARG;
+    if(id < 0 || id >= (int64_t)B->jobs.size() || !B->have[(size_t)id]) return FP_ERR_NOT_FOUND;
+    *out = B->results[(size_t)id];
+    return FP_OK;
+}
+
+void fp_batch_stats(fp_batch *B, int64_t *steps, int64_t *fluid_updates, double *seconds)
+{
+    if(!B) return;
+    if(steps) *steps = B->b->stats().steps;
+    if(fluid_updates) *fluid_updates = B->updates;
+    if(seconds) *seconds = B->seconds;
+}
+
+void fp_batch_set_chunk(fp_batch *B, int steps)
+{
+    if(B) B->b->set_chunk(steps);
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// benchmark entry: time stepping of one resident lattice
+// ---------------------------------------------------------------------------
+extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
+                              int precision, const double F[3], int warmup, int steps,
+                              fp_bench_out *out)
+{
+    if(!ctx || !solid || !out || (lattice != 2 && lattice != 3) || steps < 2 || warmup < 0)
+        return FP_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    (void)hipSetDevice(ctx->device);
+    Geometry g;
+    std::string e = geometry_build(g, solid, lattice, nx, ny, lattice == 3 ? nz : 1);
+    if(!e.empty()) return fail(ctx, status_of_geometry_error(e), e);
+    Job j;
+    j.geo = &g;
+    j.lattice = lattice;
+    j.prec = precision;
+    for(int k = 0; k < 3; k++) j.F[k] = F[k];
+    if(lattice == 3) {
+        fp_config3d c;
+        fp_config3d_default(&c);
+        c.eps = 0.0;                                  // never converges: pure stepping
+        c.max_steps = INT64_MAX / 4;
+        j.cfg = cfg3(c, g.nn, false);
+    } else {
+        fp_config2d c;
+        fp_config2d_default(&c);
+        c.eps = 0.0; c.ksigma = 0.0; c.qfloor = 0.0;
+        c.max_steps = INT64_MAX / 4;
+        j.cfg = cfg2(c, nx, ny);
+    }
+    Batch b(lattice, precision, ctx->stream, 2, false);
+    e = b.admit(&j);
+    if(!e.empty()) return fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
+    Batch::KernelTimes kw, kt;
+    if(warmup >= 2) {
+        e = b.time_steps(warmup, kw);
+        if(!e.empty()) { b.detach_all(); free_job_memory(j); return fail(ctx, FP_ERR_DEVICE, e); }
+    }
+    e = b.time_steps(steps, kt);
+    b.detach_all();
+    free_job_memory(j);
+    if(!e.empty()) return fail(ctx, FP_ERR_DEVICE, e);
+    out->n_fluid = g.n_fluid;
+    out->n_cells = g.nn;
+    out->steps = steps & ~1;
+    out->total_ms = kt.total_ms;
+    out->local_ms = kt.n_local ? kt.local_ms / kt.n_local : 0;
+    out->nbr_ms = kt.n_nbr ? kt.nbr_ms / kt.n_nbr : 0;
+    out->monitor_ms = kt.n_monitor ? kt.monitor_ms / kt.n_monitor : 0;
+    out->bytes_per_update = (double)(lattice == 3 ? 19 : 9) * 2 * (precision == FP_F64 ? 8 : 4);
+    return FP_OK;
+}

@@ -1,0 +1,127 @@
+// engine.h -- host side of the solver: structure geometry on the device,
+// forced solves ("jobs"), and the batch that steps many jobs together.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "lbm_common.h"
+
+namespace fp {
+
+// kernels.hip
+void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
+void kern_step(int lattice, int prec, bool local, const InstDesc *insts, const InstState *st,
+               const void *work, int nwork, Partial *part, hipStream_t s);
+void kern_monitor(int ninst, const InstDesc *insts, const InstCfg *cfgs, InstState *st,
+                  const Partial *lpart, const Partial *npart, hipStream_t s);
+
+// One voxel structure resident on the device: the segment table.
+struct Geometry {
+    int dim = 3, nx = 0, ny = 0, nz = 1, nseg = 0;
+    int64_t nrows = 0, nunits = 0, n_fluid = 0, nn = 0;
+    SegInfo *d_seg = nullptr;
+    std::vector<uint8_t> solid;      // host copy, x fastest
+    int refs = 0;                    // jobs using it
+    ~Geometry();
+    size_t device_bytes() const { return (size_t)nunits * sizeof(SegInfo); }
+};
+
+// builds the segment table from a mask (x fastest; 1 = solid) and uploads it.
+// Returns an empty string or the error.
+std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz);
+
+// One forced solve.
+struct Job {
+    Geometry *geo = nullptr;
+    int lattice = 3, prec = PREC_F64;
+    double F[3] = {0, 0, 0};
+    InstCfg cfg{};
+    int64_t stride = 0;
+    void *d_f = nullptr;
+    float *d_vel[2] = {nullptr, nullptr};
+    bool want_fields = false;        // keep the velocity fields (3D export, 2D fieldres)
+    InstState state{};
+    bool admitted = false, done = false;
+    int64_t id = -1;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    void *user = nullptr;
+    size_t device_bytes() const;
+};
+
+// bytes of device memory a job of this shape needs
+size_t job_bytes(const Geometry &g, int lattice, int prec, bool fields);
+
+struct BatchStats {
+    int64_t steps = 0;               // lattice steps launched (chunks * chunk length)
+    int64_t chunks = 0;
+    double  gpu_ms = 0;              // time inside chunks
+};
+
+// A set of jobs of one (lattice, precision) stepped together.  Jobs join and
+// leave only between chunks; a chunk is `chunk` steps (even) captured once in
+// a hipGraph and replayed while the membership stays the same.
+class Batch {
+public:
+    Batch(int lattice, int prec, hipStream_t stream, int chunk = 200, bool use_graph = true);
+    ~Batch();
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+
+    // allocate the job's lattice and put it at rest; it joins at the next chunk
+    std::string admit(Job *j);
+    // run one chunk; returns an error string (device failure) or empty
+    std::string run_chunk();
+    // jobs that finished (or failed) since the last call; the caller reads
+    // their fields and then calls release()
+    std::vector<Job *> take_done();
+    void release(Job *j);
+    // forget every job without freeing its buffers (the caller owns them now)
+    void detach_all();
+    int live() const { return (int)jobs_.size() + (int)pending_.size(); }
+    int lattice() const { return lattice_; }
+    int prec() const { return prec_; }
+    const BatchStats &stats() const { return stats_; }
+    void set_chunk(int c) { chunk_ = c < 2 ? 2 : (c & ~1); dirty_ = true; }
+
+    // Timing: launch `steps` steps (even) directly, with a HIP event pair on
+    // the batch stream around every kernel; returns the summed kernel times
+    // per kind and the wall time of the whole region (events too).
+    struct KernelTimes {
+        double total_ms = 0, local_ms = 0, nbr_ms = 0, monitor_ms = 0;
+        int64_t n_local = 0, n_nbr = 0, n_monitor = 0;
+    };
+    std::string time_steps(int steps, KernelTimes &kt);
+
+private:
+    std::string rebuild();
+    void release_graph();
+    void free_job(Job *j);
+
+    int lattice_, prec_;
+    hipStream_t stream_;
+    int chunk_;
+    bool use_graph_;
+    bool dirty_ = true;
+    std::vector<Job *> jobs_, pending_, finished_;
+    // device tables
+    InstDesc *d_inst_ = nullptr;
+    InstCfg *d_cfg_ = nullptr;
+    InstState *d_state_ = nullptr;
+    LocalWork *d_lwork_ = nullptr;
+    NbrWork *d_nwork_ = nullptr;
+    Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
+    InstState *h_state_ = nullptr;       // pinned mirror
+    int n_inst_ = 0, n_lwork_ = 0, n_nwork_ = 0;
+    size_t cap_inst_ = 0, cap_lwork_ = 0, cap_nwork_ = 0, cap_lpart_ = 0, cap_npart_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    BatchStats stats_;
+};
+
+// the D3Q19 FP32 per-direction force terms of InstDesc
+void f32_force_terms(const double F[3], float g45[19], float h15[19]);
+
+double now_seconds();
+
+} // namespace fp

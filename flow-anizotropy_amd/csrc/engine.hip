@@ -1,0 +1,400 @@
+// engine.hip -- geometry upload, job lattices and the chunked batch stepper.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace fp {
+
+double now_seconds()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static std::string hip_err(const char *what, hipError_t e)
+{
+    return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+Geometry::~Geometry()
+{
+    if(d_seg) (void)hipFree(d_seg);
+}
+
+std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz)
+{
+    if(nx <= 0 || ny <= 0 || (dim == 3 && nz <= 0)) return "bad dimensions";
+    g.dim = dim;
+    g.nx = nx; g.ny = ny; g.nz = dim == 3 ? nz : 1;
+    g.nseg = (nx + 63) / 64;
+    g.nrows = (int64_t)g.ny * g.nz;
+    g.nunits = g.nrows * g.nseg;
+    g.nn = (int64_t)nx * g.nrows;
+    if(g.nunits > INT32_MAX) return "lattice too large";
+    std::vector<SegInfo> h((size_t)g.nunits);
+    uint64_t count = 0;
+    for(int64_t row = 0; row < g.nrows; row++) {
+        const uint8_t *r = solid + row * nx;
+        for(int sg = 0; sg < g.nseg; sg++) {
+            uint64_t m = 0;
+            const int x0 = sg * 64, x1 = std::min(nx, x0 + 64);
+            for(int x = x0; x < x1; x++) {
+                if(r[x] > 1) return "non-binary voxel";
+                if(r[x] == 0) m |= 1ull << (x - x0);
+            }
+            SegInfo &s = h[(size_t)(row * g.nseg + sg)];
+            s.mask = m;
+            s.start = (uint32_t)count;
+            s.pad = 0;
+            count += (uint64_t)__builtin_popcountll(m);
+        }
+    }
+    if(count == 0) return "structure has no pore space";
+    if(count >= (1ull << 32)) return "too many fluid cells for 32-bit indices";
+    g.n_fluid = (int64_t)count;
+    hipError_t e = hipMalloc(&g.d_seg, h.size() * sizeof(SegInfo));
+    if(e != hipSuccess) { g.d_seg = nullptr; return hip_err("hipMalloc(segments)", e); }
+    e = hipMemcpy(g.d_seg, h.data(), h.size() * sizeof(SegInfo), hipMemcpyHostToDevice);
+    if(e != hipSuccess) return hip_err("hipMemcpy(segments)", e);
+    g.solid.assign(solid, solid + g.nn);
+    return "";
+}
+
+static int lattice_q(int lattice) { return lattice == 3 ? 19 : 9; }
+static size_t elem_bytes(int prec) { return prec == PREC_F64 ? 8 : 4; }
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+size_t job_bytes(const Geometry &g, int lattice, int prec, bool fields)
+{
+    const int64_t stride = round_up(g.n_fluid, 64);
+    size_t b = (size_t)lattice_q(lattice) * stride * elem_bytes(prec);
+    if(fields) b += 2 * (size_t)g.dim * g.n_fluid * sizeof(float);
+    return b;
+}
+
+size_t Job::device_bytes() const { return geo ? job_bytes(*geo, lattice, prec, want_fields) : 0; }
+
+void f32_force_terms(const double F[3], float g45[19], float h15[19])
+{
+    const float Ff[3] = {(float)F[0], (float)F[1], (float)F[2]};
+    for(int k = 0; k < 19; k++) {
+        const float w = (float)D3Q19::w(k);
+        const float ef = D3Q19::ex(k) * Ff[0] + D3Q19::ey(k) * Ff[1] + D3Q19::ez(k) * Ff[2];
+        g45[k] = 4.5f * w * ef;
+        h15[k] = 1.5f * w * ef;
+    }
+}
+
+static InstDesc make_desc(const Job &j)
+{
+    InstDesc D;
+    std::memset(&D, 0, sizeof(D));
+    const Geometry &g = *j.geo;
+    D.seg = g.d_seg;
+    D.f = j.d_f;
+    D.vel[0] = j.d_vel[0];
+    D.vel[1] = j.d_vel[1];
+    D.stride = j.stride;
+    D.n_fluid = g.n_fluid;
+    D.nx = g.nx; D.ny = g.ny; D.nz = g.nz; D.nseg = g.nseg;
+    for(int k = 0; k < 3; k++) { D.F[k] = j.F[k]; D.Ff[k] = (float)j.F[k]; }
+    D.lattice = j.lattice;
+    D.prec = j.prec;
+    if(j.lattice == 3) f32_force_terms(j.F, D.g45, D.h15);
+    return D;
+}
+
+// ---------------------------------------------------------------------------
+
+Batch::Batch(int lattice, int prec, hipStream_t stream, int chunk, bool use_graph)
+    : lattice_(lattice), prec_(prec), stream_(stream), chunk_(chunk < 2 ? 2 : (chunk & ~1)),
+      use_graph_(use_graph) {}
+
+Batch::~Batch()
+{
+    (void)hipStreamSynchronize(stream_);
+    release_graph();
+    for(auto *list : {&jobs_, &pending_, &finished_})
+        for(Job *j : *list) {
+            if(!j) continue;
+            free_job(j);
+            if(j->admitted) { j->geo->refs--; j->admitted = false; }
+        }
+    (void)hipFree(d_inst_);
+    (void)hipFree(d_cfg_);
+    (void)hipFree(d_state_);
+    (void)hipFree(d_lwork_);
+    (void)hipFree(d_nwork_);
+    (void)hipFree(d_lpart_);
+    (void)hipFree(d_npart_);
+    if(h_state_) (void)hipHostFree(h_state_);
+}
+
+void Batch::release_graph()
+{
+    if(exec_) { (void)hipGraphExecDestroy(exec_); exec_ = nullptr; }
+    if(graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+}
+
+void Batch::free_job(Job *j)
+{
+    if(j->d_f) { (void)hipFree(j->d_f); j->d_f = nullptr; }
+    for(int s = 0; s < 2; s++)
+        if(j->d_vel[s]) { (void)hipFree(j->d_vel[s]); j->d_vel[s] = nullptr; }
+}
+
+std::string Batch::admit(Job *j)
+{
+    if(j->lattice != lattice_ || j->prec != prec_) return "job does not match the batch type";
+    const Geometry &g = *j->geo;
+    j->stride = round_up(g.n_fluid, 64);
+    const size_t fb = (size_t)lattice_q(lattice_) * j->stride * elem_bytes(prec_);
+    hipError_t e = hipMalloc(&j->d_f, fb);
+    if(e != hipSuccess) { j->d_f = nullptr; (void)hipGetLastError(); return "out of device memory"; }
+    if(j->want_fields) {
+        for(int s = 0; s < 2; s++) {
+            e = hipMalloc(&j->d_vel[s], (size_t)g.dim * g.n_fluid * sizeof(float));
+            if(e != hipSuccess) {
+                j->d_vel[s] = nullptr;
+                (void)hipGetLastError();
+                free_job(j);
+                return "out of device memory";
+            }
+        }
+    }
+    j->geo->refs++;
+    kern_rest(lattice_, prec_, make_desc(*j), stream_);
+    e = hipGetLastError();
+    if(e != hipSuccess) return hip_err("rest-state kernel", e);
+    init_state(j->cfg, j->state);
+    j->admitted = true;
+    j->done = j->state.phase >= PH_DONE;
+    j->t_admit = now_seconds();
+    if(j->done) { j->t_done = j->t_admit; finished_.push_back(j); }
+    else { pending_.push_back(j); dirty_ = true; }
+    return "";
+}
+
+template<class T>
+static hipError_t ensure(T *&p, size_t &cap, size_t n)
+{
+    if(n <= cap && p) return hipSuccess;
+    if(p) (void)hipFree(p);
+    p = nullptr;
+    size_t c = std::max<size_t>(n, cap * 3 / 2 + 1);
+    hipError_t e = hipMalloc(&p, c * sizeof(T));
+    if(e == hipSuccess) cap = c; else { p = nullptr; cap = 0; }
+    return e;
+}
+
+std::string Batch::rebuild()
+{
+    hipError_t e = hipStreamSynchronize(stream_);
+    if(e != hipSuccess) return hip_err("stream", e);
+    release_graph();
+    // drop finished jobs from the table, take in the pending ones
+    std::vector<Job *> live;
+    for(Job *j : jobs_) if(j && !j->done) live.push_back(j);
+    for(Job *j : pending_) live.push_back(j);
+    pending_.clear();
+    jobs_ = live;
+    n_inst_ = (int)jobs_.size();
+    if(n_inst_ == 0) { dirty_ = false; n_lwork_ = n_nwork_ = 0; return ""; }
+
+    std::vector<InstDesc> desc(n_inst_);
+    std::vector<InstCfg> cfg(n_inst_);
+    std::vector<InstState> state(n_inst_);
+    std::vector<LocalWork> lw;
+    std::vector<NbrWork> nw;
+    for(int i = 0; i < n_inst_; i++) {
+        Job *j = jobs_[i];
+        desc[i] = make_desc(*j);
+        cfg[i] = j->cfg;
+        state[i] = j->state;
+        const Geometry &g = *j->geo;
+        desc[i].lp0 = (int32_t)lw.size();
+        for(int64_t c = 0; c < g.n_fluid; c += 256) {
+            LocalWork w;
+            w.inst = i;
+            w.ncell = (int32_t)std::min<int64_t>(256, g.n_fluid - c);
+            w.first = c;
+            lw.push_back(w);
+        }
+        desc[i].lp1 = (int32_t)lw.size();
+        desc[i].np0 = (int32_t)nw.size();
+        for(int64_t u = 0; u < g.nunits; u += 4) {
+            NbrWork w;
+            w.inst = i;
+            w.nunit = (int32_t)std::min<int64_t>(4, g.nunits - u);
+            w.first = u;
+            nw.push_back(w);
+        }
+        desc[i].np1 = (int32_t)nw.size();
+    }
+    if(lw.size() > INT32_MAX || nw.size() > INT32_MAX) return "batch too large";
+    n_lwork_ = (int)lw.size();
+    n_nwork_ = (int)nw.size();
+    const size_t ni = (size_t)n_inst_;
+    if(ni > cap_inst_ || !d_inst_) {
+        (void)hipFree(d_inst_); (void)hipFree(d_cfg_); (void)hipFree(d_state_);
+        d_inst_ = nullptr; d_cfg_ = nullptr; d_state_ = nullptr;
+        if(h_state_) { (void)hipHostFree(h_state_); h_state_ = nullptr; }
+        cap_inst_ = 0;
+        const size_t c = std::max(ni, (size_t)16);
+        if((e = hipMalloc(&d_inst_, c * sizeof(InstDesc))) != hipSuccess ||
+           (e = hipMalloc(&d_cfg_, c * sizeof(InstCfg))) != hipSuccess ||
+           (e = hipMalloc(&d_state_, c * sizeof(InstState))) != hipSuccess ||
+           (e = hipHostMalloc((void **)&h_state_, c * sizeof(InstState), hipHostMallocDefault)) != hipSuccess)
+            return hip_err("allocating batch tables", e);
+        cap_inst_ = c;
+    }
+    if((e = ensure(d_lwork_, cap_lwork_, std::max<size_t>(1, lw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
+    if((e = ensure(d_nwork_, cap_nwork_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
+    if((e = ensure(d_lpart_, cap_lpart_, std::max<size_t>(1, lw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
+    if((e = ensure(d_npart_, cap_npart_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
+    e = hipMemcpy(d_inst_, desc.data(), desc.size() * sizeof(InstDesc), hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(d_cfg_, cfg.data(), cfg.size() * sizeof(InstCfg), hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(d_state_, state.data(), state.size() * sizeof(InstState), hipMemcpyHostToDevice);
+    if(e == hipSuccess && !lw.empty()) e = hipMemcpy(d_lwork_, lw.data(), lw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);
+    if(e == hipSuccess && !nw.empty()) e = hipMemcpy(d_nwork_, nw.data(), nw.size() * sizeof(NbrWork), hipMemcpyHostToDevice);
+    if(e != hipSuccess) return hip_err("hipMemcpy(tables)", e);
+    dirty_ = false;
+    return "";
+}
+
+std::string Batch::run_chunk()
+{
+    if(dirty_) {
+        std::string err = rebuild();
+        if(!err.empty()) return err;
+    }
+    if(n_inst_ == 0) return "";
+    const double t0 = now_seconds();
+    hipError_t e;
+    auto launch_all = [&]() {
+        for(int t = 0; t < chunk_; t++) {
+            const bool local = (t & 1) == 0;     // every live job starts a chunk at an even step count
+            kern_step(lattice_, prec_, local, d_inst_, d_state_, local ? (const void *)d_lwork_ : (const void *)d_nwork_,
+                      local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
+            kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lpart_, d_npart_, stream_);
+        }
+    };
+    if(use_graph_) {
+        if(!exec_) {
+            e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+            if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
+            launch_all();
+            e = hipStreamEndCapture(stream_, &graph_);
+            if(e != hipSuccess) { graph_ = nullptr; return hip_err("hipStreamEndCapture", e); }
+            e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+            if(e != hipSuccess) { exec_ = nullptr; return hip_err("hipGraphInstantiate", e); }
+        }
+        e = hipGraphLaunch(exec_, stream_);
+        if(e != hipSuccess) return hip_err("hipGraphLaunch", e);
+    } else {
+        launch_all();
+        e = hipGetLastError();
+        if(e != hipSuccess) return hip_err("kernel launch", e);
+    }
+    e = hipMemcpyAsync(h_state_, d_state_, n_inst_ * sizeof(InstState), hipMemcpyDeviceToHost, stream_);
+    if(e != hipSuccess) return hip_err("hipMemcpyAsync(state)", e);
+    e = hipStreamSynchronize(stream_);
+    if(e != hipSuccess) return hip_err("device", e);
+    stats_.steps += chunk_;
+    stats_.chunks += 1;
+    stats_.gpu_ms += (now_seconds() - t0) * 1e3;
+    const double t1 = now_seconds();
+    for(int i = 0; i < n_inst_; i++) {
+        Job *j = jobs_[i];
+        if(j->done) continue;
+        j->state = h_state_[i];
+        if(j->state.phase >= PH_DONE) {
+            j->done = true;
+            j->t_done = t1;
+            finished_.push_back(j);
+            dirty_ = true;
+        }
+    }
+    return "";
+}
+
+std::string Batch::time_steps(int steps, KernelTimes &kt)
+{
+    if(dirty_) {
+        std::string err = rebuild();
+        if(!err.empty()) return err;
+    }
+    if(n_inst_ == 0) return "nothing to step";
+    steps &= ~1;
+    const int nev = 4 * steps + 2;
+    std::vector<hipEvent_t> ev(nev);
+    for(auto &x : ev)
+        if(hipEventCreate(&x) != hipSuccess) return "hipEventCreate failed";
+    int k = 0;
+    (void)hipEventRecord(ev[k++], stream_);
+    for(int t = 0; t < steps; t++) {
+        const bool local = (t & 1) == 0;
+        (void)hipEventRecord(ev[k++], stream_);
+        kern_step(lattice_, prec_, local, d_inst_, d_state_, local ? (const void *)d_lwork_ : (const void *)d_nwork_,
+                  local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
+        (void)hipEventRecord(ev[k++], stream_);
+        kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lpart_, d_npart_, stream_);
+        (void)hipEventRecord(ev[k++], stream_);
+        (void)hipEventRecord(ev[k++], stream_);
+    }
+    (void)hipEventRecord(ev[k++], stream_);
+    hipError_t e = hipStreamSynchronize(stream_);
+    if(e != hipSuccess) return hip_err("device", e);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[k - 1]);
+    kt.total_ms = ms;
+    for(int t = 0; t < steps; t++) {
+        const int b = 1 + 4 * t;
+        float a = 0, m = 0;
+        (void)hipEventElapsedTime(&a, ev[b], ev[b + 1]);
+        (void)hipEventElapsedTime(&m, ev[b + 1], ev[b + 2]);
+        if((t & 1) == 0) { kt.local_ms += a; kt.n_local++; }
+        else             { kt.nbr_ms += a; kt.n_nbr++; }
+        kt.monitor_ms += m;
+        kt.n_monitor++;
+    }
+    for(auto &x : ev) (void)hipEventDestroy(x);
+    // pull the states back so the host mirror stays current
+    e = hipMemcpy(h_state_, d_state_, n_inst_ * sizeof(InstState), hipMemcpyDeviceToHost);
+    if(e != hipSuccess) return hip_err("hipMemcpy(state)", e);
+    for(int i = 0; i < n_inst_; i++) if(jobs_[i]) jobs_[i]->state = h_state_[i];
+    return "";
+}
+
+std::vector<Job *> Batch::take_done()
+{
+    std::vector<Job *> out;
+    out.swap(finished_);
+    return out;
+}
+
+void Batch::detach_all()
+{
+    (void)hipStreamSynchronize(stream_);
+    for(auto *list : {&jobs_, &pending_, &finished_})
+        for(Job *j : *list)
+            if(j && j->admitted) { j->geo->refs--; j->admitted = false; }
+    jobs_.clear();
+    pending_.clear();
+    finished_.clear();
+    dirty_ = true;
+}
+
+void Batch::release(Job *j)
+{
+    // the tables still name the job until the next rebuild, but its phase is
+    // final, so no kernel reads its lattice; mark the tables stale
+    free_job(j);
+    if(j->admitted) { j->geo->refs--; j->admitted = false; }
+    for(Job *&p : jobs_) if(p == j) p = nullptr;     // the caller may now delete it
+    dirty_ = true;
+}
+
+} // namespace fp

@@ -1,0 +1,420 @@
+// lbm_common.h -- lattice tables, exact flux arithmetic and the device-side
+// descriptors shared by the D3Q19 / D2Q9 kernels and the host engine.
+//
+// Lattices (population order is the reference's and fixes every array index):
+//   D3Q19: LMB3d/lbm3d-perm.cpp:94-110  (rest, 6 faces, 12 edges)
+//   D2Q9 : LMB2d/lbm.cpp:20-23
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FP_HD __host__ __device__ __forceinline__
+
+namespace fp {
+
+// ---------------------------------------------------------------------------
+// lattice tables
+// ---------------------------------------------------------------------------
+struct D3Q19 {
+    static constexpr int Q = 19;
+    static constexpr int DIM = 3;
+    static FP_HD int ex(int k) { constexpr int v[19] = {0, 1,-1, 0, 0, 0, 0, 1,-1, 1,-1, 1,-1, 1,-1, 0, 0, 0, 0}; return v[k]; }
+    static FP_HD int ey(int k) { constexpr int v[19] = {0, 0, 0, 1,-1, 0, 0, 1,-1,-1, 1, 0, 0, 0, 0, 1,-1, 1,-1}; return v[k]; }
+    static FP_HD int ez(int k) { constexpr int v[19] = {0, 0, 0, 0, 0, 1,-1, 0, 0, 0, 0, 1,-1,-1, 1, 1,-1,-1, 1}; return v[k]; }
+    // opp[k]: the reference finds it by search (:105-110); the result is this table
+    static FP_HD int opp(int k) { constexpr int v[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17}; return v[k]; }
+    static FP_HD double w(int k) {
+        return k == 0 ? 1.0/3.0 : (k < 7 ? 1.0/18.0 : 1.0/36.0);
+    }
+};
+
+struct D2Q9 {
+    static constexpr int Q = 9;
+    static constexpr int DIM = 2;
+    static FP_HD int ex(int k) { constexpr int v[9] = {0, 1, 0,-1, 0, 1,-1,-1, 1}; return v[k]; }
+    static FP_HD int ey(int k) { constexpr int v[9] = {0, 0, 1, 0,-1, 1, 1,-1,-1}; return v[k]; }
+    static FP_HD int ez(int) { return 0; }
+    static FP_HD int opp(int k) { constexpr int v[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6}; return v[k]; }
+    // the reference keeps w as float (lbm.cpp:23); every 2D variant uses these values
+    static FP_HD float wf(int k) {
+        return k == 0 ? (float)(4.0/9.0) : (k < 5 ? (float)(1.0/9.0) : (float)(1.0/36.0));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// exact flux sums
+//
+// A node velocity u (|u| < 1) contributes trunc(u * 2^96) to a 128-bit
+// integer sum.  Integer addition is associative, so the total does not depend
+// on block shapes, launch order or batch composition; it is rounded once to
+// double.  On the device u is split exactly into three integer-valued doubles
+//   u*2^96 = c0*2^88 + c1*2^44 + c2,  |c0| < 2^8, |c1|,|c2| < 2^44,
+// all of the sign of u, which a wavefront adds exactly in double (64 terms of
+// < 2^44 stay below 2^53) before they are widened to 64-bit integers.
+// ---------------------------------------------------------------------------
+struct I128 {
+    uint64_t lo;
+    int64_t  hi;
+};
+
+FP_HD I128 i128_zero() { I128 r; r.lo = 0; r.hi = 0; return r; }
+
+FP_HD I128 i128_from_i64(int64_t v) { I128 r; r.lo = (uint64_t)v; r.hi = v < 0 ? -1 : 0; return r; }
+
+FP_HD I128 i128_add(I128 a, I128 b)
+{
+    I128 r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1 : 0);
+    return r;
+}
+
+// v * 2^sh for 0 <= sh < 64
+FP_HD I128 i128_shl(int64_t v, int sh)
+{
+    I128 r = i128_from_i64(v);
+    if(sh == 0) return r;
+    I128 o;
+    o.lo = r.lo << sh;
+    o.hi = (int64_t)(((uint64_t)r.hi << sh) | (r.lo >> (64 - sh)));
+    return o;
+}
+
+// the three chunk sums of a block (or of a whole lattice) as one integer
+FP_HD I128 i128_from_chunks(int64_t c0, int64_t c1, int64_t c2)
+{
+    // c0 * 2^88 = (c0 * 2^24) * 2^64
+    I128 a; a.lo = 0; a.hi = c0 * (int64_t)(1 << 24);
+    return i128_add(i128_add(a, i128_shl(c1, 44)), i128_from_i64(c2));
+}
+
+// round-to-nearest-even conversion of a 128-bit integer to double, then *2^-96
+FP_HD double i128_to_double_scaled(I128 s)
+{
+    const bool neg = s.hi < 0;
+    uint64_t hi = (uint64_t)s.hi, lo = s.lo;
+    if(neg) {                                    // magnitude
+        lo = ~lo + 1;
+        hi = ~hi + (lo == 0 ? 1 : 0);
+    }
+    if(hi == 0 && lo == 0) return 0.0;
+    int lz = hi ? __builtin_clzll(hi) : 64 + __builtin_clzll(lo);
+    int top = 127 - lz;                          // index of the leading one
+    // take 53 significant bits m = floor(|s| / 2^(top-52)), rest for rounding
+    uint64_t m;
+    bool round_up = false;
+    int e = top - 52;                            // |s| ~ m * 2^e
+    if(e <= 0) {
+        m = lo << (-e);                          // exact (top <= 52, hi == 0)
+    } else {
+        // shift right by e (1..75)
+        uint64_t mm, rem_hi_bit, sticky;
+        if(e < 64) {
+            mm = (lo >> e) | (e ? (hi << (64 - e)) : 0);
+            rem_hi_bit = (lo >> (e - 1)) & 1u;
+            uint64_t below = (e - 1) ? (lo & ((1ull << (e - 1)) - 1)) : 0;
+            sticky = below != 0;
+        } else {
+            int e2 = e - 64;
+            mm = hi >> e2;
+            if(e2 == 0) {
+                rem_hi_bit = lo >> 63;
+                sticky = (lo & ((1ull << 63) - 1)) != 0;
+            } else {
+                rem_hi_bit = (hi >> (e2 - 1)) & 1u;
+                uint64_t below_hi = (e2 - 1) ? (hi & ((1ull << (e2 - 1)) - 1)) : 0;
+                sticky = (below_hi | lo) != 0;
+            }
+        }
+        m = mm;
+        round_up = rem_hi_bit && (sticky || (m & 1u));
+    }
+    if(round_up) {
+        m += 1;
+        if(m == (1ull << 53)) { m >>= 1; e += 1; }
+    }
+    double d = (double)m;                        // exact: m < 2^53
+    d = ldexp(d, e - 96);
+    return neg ? -d : d;
+}
+
+// split u*2^96 into three integer-valued doubles; *bad when |u| >= 1 or NaN
+FP_HD void fix_split(double u, double &c0, double &c1, double &c2, int &bad)
+{
+    if(!(fabs(u) < 1.0)) { bad = 1; c0 = c1 = c2 = 0.0; return; }
+    double a = u * 256.0;                        // 2^8
+    c0 = trunc(a);
+    double r = (a - c0) * 17592186044416.0;      // 2^44
+    c1 = trunc(r);
+    double r2 = (r - c1) * 17592186044416.0;
+    c2 = trunc(r2);
+}
+
+// ---------------------------------------------------------------------------
+// device descriptors
+// ---------------------------------------------------------------------------
+// one 64-cell run of a lattice row along x: fluid bits and the compact index
+// of its first fluid cell.  Rows are ordered z-major then y, x fastest, the
+// .raw order, so compact indices follow the reference's node order.
+struct SegInfo {
+    uint64_t mask;
+    uint32_t start;
+    uint32_t pad;
+};
+
+enum Prec : int {
+    PREC_F64 = 0,       // f in double, the reference's arithmetic
+    PREC_F32 = 1,       // f - w in float (shifted), FP64 flux sums
+    PREC_F32_REF2D = 2  // D2Q9 only: unshifted float, the 2D reference's arithmetic
+};
+
+struct InstDesc {
+    const SegInfo *seg;     // [nrows * nseg]
+    void    *f;             // populations f[k * stride + i]
+    float   *vel[2];        // compact velocity fields (DIM floats per cell), or null
+    int64_t  stride;        // elements per population array
+    int64_t  n_fluid;
+    int      nx, ny, nz, nseg;
+    double   F[3];          // body force
+    float    Ff[3];         // the same as float (2D reference stores fx as float)
+    int      lattice;       // 3 = D3Q19, 2 = D2Q9
+    int      prec;
+    // FP32 D3Q19 per-direction force terms 4.5 w_k (e_k.F) and 1.5 w_k (e_k.F)
+    float    g45[19], h15[19];
+    // partial-sum slots of this instance in the local / neighbour kernels
+    int32_t  lp0, lp1, np0, np1;
+};
+
+// convergence state of one solve, advanced on the device after every step.
+// Phase 1 / phase 2 follow LMB3d/lbm3d-perm.cpp:320-398 (kind 3) or
+// LMB2d/main.cpp:99-235 (kind 2).
+struct InstCfg {
+    double  eps, qfloor, ksigma;
+    int64_t lag, min_steps, max_steps, avg_steps, avg_sample;
+    int     hold;
+    int     kind;           // 3 or 2
+    int     want_vel;       // write the velocity field on the final step
+    int     pad;
+    double  denom[3];       // q_c = sum_c / denom[c]  (NN in 3D; L*L0*L0 in 2D)
+};
+
+enum Phase : int { PH_RUN = 0, PH_AVG = 1, PH_DONE = 2, PH_FAILED = 3, PH_IDLE = 4 };
+
+struct InstState {
+    int64_t s;              // steps done
+    int64_t tavg;           // steps done in the averaging phase
+    int     phase;
+    int     hold;
+    int     have_prev;
+    int     flux_next;      // the next step must produce the flux sums
+    int     vel_next;       // the next step writes the velocity field: 0 no, 1 slot A, 2 slot B
+    int     converged;
+    int64_t fail_step;
+    int64_t n;              // samples in the averaging phase
+    double  qprev[3];
+    double  sum[3], sumsq[3];
+    // 2D block statistics (main.cpp:119-125)
+    double  bs[2], bss[2], pm[2], pse[2];
+    int64_t bn;
+    double  q_last[3];      // flux of the last flux step
+    // results
+    double  q[3], se[3];
+    int64_t steps;
+    int     status;         // 0 ok, 2 non-finite flux
+    int     pad;
+};
+
+// ---------------------------------------------------------------------------
+// the convergence state machine, shared by the device monitor and the host
+// ---------------------------------------------------------------------------
+FP_HD double dmax_ref(double a, double b) { return (a < b) ? b : a; }   // std::max
+
+// which outputs the next step must produce
+FP_HD void plan_next(const InstCfg &c, InstState &S)
+{
+    S.flux_next = 0;
+    S.vel_next = 0;
+    if(S.phase == PH_RUN) {
+        const int64_t t = S.s + 1;
+        const bool maybe_last = (t % c.lag == 0) || (t >= c.max_steps);
+        if(c.kind == 3) {
+            S.flux_next = (t % c.lag == 0) || (c.avg_steps <= 0 && t >= c.max_steps);
+            if(c.want_vel && c.avg_steps <= 0 && maybe_last) S.vel_next = 2;
+        } else {
+            S.flux_next = (t % c.avg_sample == 0) || (c.avg_steps <= 0 && maybe_last);
+            if(maybe_last) S.vel_next = 1;               // field snapshot candidate
+        }
+    } else if(S.phase == PH_AVG) {
+        const int64_t ta = S.tavg + 1;
+        const bool last = ta >= c.avg_steps;
+        if(c.kind == 3) {
+            S.flux_next = 1;
+            if(c.want_vel && last) S.vel_next = 2;
+        } else {
+            S.flux_next = (ta % c.avg_sample == 0) || (last && c.avg_steps < c.avg_sample);
+            if(last) S.vel_next = 2;
+        }
+    }
+}
+
+// solve(), end of the run: LMB3d/lbm3d-perm.cpp:380-392 / LMB2d/main.cpp:210-228
+FP_HD void finish_state(const InstCfg &c, InstState &S)
+{
+    const int nd = c.kind == 3 ? 3 : 2;
+    if(S.n == 0) {
+        for(int k = 0; k < nd; k++) S.q[k] = S.q_last[k];
+    } else {
+        for(int k = 0; k < nd; k++) {
+            S.q[k] = S.sum[k] / S.n;
+            if(S.n > 1) {
+                const double var = (S.sumsq[k] - S.n*S.q[k]*S.q[k]) / (S.n - 1);
+                S.se[k] = sqrt(dmax_ref(var, 0.0) / S.n);
+            }
+        }
+    }
+    S.steps = S.s;
+    S.phase = PH_DONE;
+}
+
+FP_HD void enter_avg(const InstCfg &c, InstState &S)
+{
+    S.phase = PH_AVG;
+    S.tavg = 0;
+    S.n = 0;
+    for(int k = 0; k < 3; k++) { S.sum[k] = 0.0; S.sumsq[k] = 0.0; }
+    if(c.avg_steps <= 0) finish_state(c, S);
+}
+
+FP_HD void fail_state(InstState &S)
+{
+    S.status = 2;
+    S.phase = PH_FAILED;
+    S.fail_step = S.s;
+    S.steps = S.s;
+}
+
+FP_HD bool all_finite(const double *q, int nd)
+{
+    for(int k = 0; k < nd; k++) if(!isfinite(q[k])) return false;
+    return true;
+}
+
+// advance by the step just taken; q holds its flux when have_q
+FP_HD void advance_state(const InstCfg &c, InstState &S, const double *q, int have_q)
+{
+    S.s += 1;
+    const int64_t s = S.s;
+    const int nd = c.kind == 3 ? 3 : 2;
+    if(have_q) for(int k = 0; k < nd; k++) S.q_last[k] = q[k];
+
+    if(c.kind == 3) {                                  // lbm3d-perm.cpp:332-390
+        if(S.phase == PH_RUN) {
+            bool end = false;
+            if(s % c.lag == 0) {
+                if(!all_finite(q, 3)) { fail_state(S); return; }
+                const double norm = sqrt(q[0]*q[0] + q[1]*q[1] + q[2]*q[2]);
+                double drift = 0.0;
+                for(int k = 0; k < 3; k++) drift = dmax_ref(drift, fabs(q[k] - S.qprev[k]));
+                if(S.have_prev && s >= c.min_steps && drift < c.eps * norm) S.hold++;
+                else                                                       S.hold = 0;
+                for(int k = 0; k < 3; k++) S.qprev[k] = q[k];
+                S.have_prev = 1;
+                if(S.hold >= c.hold) end = true;
+            }
+            if(!end && s >= c.max_steps) end = true;
+            if(end) {
+                S.converged = (S.hold >= c.hold) ? 1 : 0;
+                enter_avg(c, S);
+            }
+        } else if(S.phase == PH_AVG) {
+            S.tavg += 1;
+            for(int k = 0; k < 3; k++) { S.sum[k] += q[k]; S.sumsq[k] += q[k]*q[k]; }
+            S.n += 1;
+            if(S.tavg >= c.avg_steps) finish_state(c, S);
+        }
+    } else {                                           // main.cpp:127-226
+        if(S.phase == PH_RUN) {
+            if(s % c.avg_sample == 0) {
+                if(!all_finite(q, 2)) { fail_state(S); return; }
+                S.bs[0] += q[0]; S.bss[0] += q[0]*q[0];
+                S.bs[1] += q[1]; S.bss[1] += q[1]*q[1];
+                S.bn += 1;
+            }
+            bool end = false;
+            if(!(s % c.lag || S.bn < 2)) {
+                const int64_t bn = S.bn;
+                const double mx = S.bs[0] / bn, my = S.bs[1] / bn;
+                const double vx = (S.bss[0] - bn*mx*mx) / (bn - 1);
+                const double vy = (S.bss[1] - bn*my*my) / (bn - 1);
+                const double sex = sqrt(dmax_ref(vx, 0.0) / bn);
+                const double sey = sqrt(dmax_ref(vy, 0.0) / bn);
+                if(S.have_prev) {
+                    const double qmag = sqrt(mx*mx + my*my);
+                    const double flo  = c.qfloor * qmag;
+                    const double dx = fabs(mx - S.pm[0]);
+                    const double dy = fabs(my - S.pm[1]);
+                    const double tolx = dmax_ref(c.eps * dmax_ref(fabs(mx), flo),
+                                                 c.ksigma * sqrt(sex*sex + S.pse[0]*S.pse[0]));
+                    const double toly = dmax_ref(c.eps * dmax_ref(fabs(my), flo),
+                                                 c.ksigma * sqrt(sey*sey + S.pse[1]*S.pse[1]));
+                    if(s >= c.min_steps && dx < tolx && dy < toly) S.hold++;
+                    else                                           S.hold = 0;
+                }
+                S.pm[0] = mx; S.pm[1] = my; S.pse[0] = sex; S.pse[1] = sey; S.have_prev = 1;
+                S.bs[0] = S.bs[1] = S.bss[0] = S.bss[1] = 0.0; S.bn = 0;
+                if(S.hold >= c.hold) end = true;
+            }
+            if(!end && s >= c.max_steps) end = true;
+            if(end) {
+                S.converged = (S.hold >= c.hold) ? 1 : 0;
+                enter_avg(c, S);
+            }
+        } else if(S.phase == PH_AVG) {
+            S.tavg += 1;
+            if(S.tavg % c.avg_sample == 0) {
+                if(!all_finite(q, 2)) { fail_state(S); return; }
+                S.sum[0] += q[0]; S.sumsq[0] += q[0]*q[0];
+                S.sum[1] += q[1]; S.sumsq[1] += q[1]*q[1];
+                S.n += 1;
+            }
+            if(S.tavg >= c.avg_steps) finish_state(c, S);
+        }
+    }
+}
+
+// state of a fresh solve
+FP_HD void init_state(const InstCfg &c, InstState &S)
+{
+    InstState z = {};
+    S = z;
+    S.phase = PH_RUN;
+    if(c.max_steps <= 0) {                  // phase 1 runs no step at all
+        S.converged = (0 >= c.hold) ? 1 : 0;
+        enter_avg(c, S);
+    }
+    plan_next(c, S);
+}
+
+// per-block exact partial sums: [component][chunk]
+struct Partial {
+    int64_t c[3][3];
+    int32_t bad;
+    int32_t pad;
+};
+
+// work of one workgroup of the local (even A-A) kernel: a run of compact cells
+struct LocalWork {
+    int32_t inst;
+    int32_t ncell;
+    int64_t first;
+};
+
+// work of one workgroup of the neighbour (odd A-A) kernel: consecutive
+// (row, segment) units of one instance, one wavefront each
+struct NbrWork {
+    int32_t inst;
+    int32_t nunit;
+    int64_t first;
+};
+
+} // namespace fp

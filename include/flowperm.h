@@ -1,0 +1,246 @@
+/*
+ * flowperm.h -- C ABI of libflowperm.so, the MI355X lattice-Boltzmann
+ * permeability solver.
+ *
+ * The reference has no library boundary: its solvers are executables driven
+ * through subprocess.run (scripts/solve_permeability3d.py:113-146,
+ * scripts/solve_permeability.py:29-46), reading a mask file and appending one
+ * CSV row, with exit codes 0 / 3 / 2 (LMB3d/lbm3d-perm.cpp:140-144, :630;
+ * LMB2d/main.cpp:426).  This ABI moves that process boundary to a host ->
+ * device boundary: masks and results cross as plain memory, and every status
+ * maps 1:1 onto the reference exit codes (fp_exit_code).
+ *
+ * Conventions
+ *  - Masks are uint8, 1 = solid, 0 = pore.  3D masks are [nz][ny][nx] with x
+ *    fastest (the .raw payload, LMB3d/lbm3d-perm.cpp:156); 2D masks are
+ *    [ly][lx], row = solver y, column = solver x (scripts/structure_io.py:44-54).
+ *  - The caller owns every input and output buffer; the library owns device
+ *    memory.  No C++ exception crosses the ABI.
+ *  - A context is used by one thread at a time; different contexts are
+ *    independent (there is no global state).
+ *  - Statuses: FP_OK / FP_NOT_CONVERGED are the reference's exit codes 0 / 3;
+ *    every FP_ERR_* is its exit code 2 (a message is kept in fp_last_error).
+ */
+#ifndef FLOWPERM_H
+#define FLOWPERM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP_ABI_VERSION 1
+
+enum fp_status {
+    FP_OK              = 0,   /* all directions converged            (exit 0) */
+    FP_NOT_CONVERGED   = 3,   /* a direction hit max_steps; row kept (exit 3) */
+    FP_ERR_BAD_DIMS    = 10,  /* read_structure: bad dimensions      (exit 2) */
+    FP_ERR_NON_BINARY  = 11,  /* read_structure: non-binary voxel    (exit 2) */
+    FP_ERR_NO_PORE     = 12,  /* read_structure: no pore space       (exit 2) */
+    FP_ERR_NONFINITE   = 13,  /* solve(): non-finite flux            (exit 2) */
+    FP_ERR_OOM         = 14,  /* device memory exhausted             (exit 2) */
+    FP_ERR_DEVICE      = 15,  /* HIP runtime failure                 (exit 2) */
+    FP_ERR_ARG         = 16,  /* invalid argument                    (exit 2) */
+    FP_ERR_NOT_FOUND   = 17   /* unknown job id                      (exit 2) */
+};
+
+enum fp_precision {
+    FP_F64        = 0,  /* populations in double: the reference's arithmetic (3D),
+                           the 2D scheme evaluated in double (2D)               */
+    FP_F32        = 1,  /* populations stored as float f - w; flux sums, the
+                           convergence test and K stay in double              */
+    FP_F32_REF2D  = 2   /* 2D only: unshifted float, the arithmetic of
+                           LMB2d/lbm.cpp:516-585 operation for operation       */
+};
+
+/* Process exit code of the reference for a status: 0, 3 or 2. */
+int fp_exit_code(int status);
+
+/* ------------------------------------------------------------------------ */
+/* contexts                                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct fp_ctx fp_ctx;
+
+/* Open a context on HIP device `device` (its own stream).  Replaces the
+ * process start of lbm3d-perm / lbm2d-perm (LMB3d/lbm3d-perm.cpp:478-481). */
+fp_ctx *fp_create(int device, int *status);
+void    fp_destroy(fp_ctx *ctx);
+/* Message of the last failure on this context (stderr text of die(),
+ * LMB3d/lbm3d-perm.cpp:140-144), or "" */
+const char *fp_last_error(fp_ctx *ctx);
+int     fp_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* 3D: D3Q19                                                                 */
+/* ------------------------------------------------------------------------ */
+/* Config + flags of LMB3d/lbm3d-perm.cpp:118-130 and :487-505. */
+typedef struct fp_config3d {
+    double  ff;             /* --ff: body force = 2.5e-7 * ff         (1)      */
+    double  eps;            /* --eps: drift tolerance relative to |q| (1e-5)   */
+    int64_t lag;            /* --lag: steps between checks            (2000)   */
+    int64_t min_steps;      /* --min-steps                            (4000)   */
+    int64_t max_steps;      /* --max-steps                            (200000) */
+    int64_t avg_steps;      /* --avg-steps                            (2000)   */
+    int32_t hold;           /* --hold                                 (2)      */
+    int32_t validate;       /* --validate: 4th solve along (1,1,1)/sqrt(3)     */
+    int32_t precision;      /* enum fp_precision                      (FP_F64) */
+    int32_t concurrent;     /* step the force directions together     (0)      */
+} fp_config3d;
+
+/* Fill with the reference defaults (LMB3d/lbm3d-perm.cpp:118-130). */
+void fp_config3d_default(fp_config3d *cfg);
+
+/* One forced solve: the Solve struct of LMB3d/lbm3d-perm.cpp:132-138. */
+typedef struct fp_solve_out {
+    double  q[3];           /* Darcy flux (superficial velocity) per axis     */
+    double  se[3];          /* standard error over the averaging phase        */
+    int64_t steps;
+    int32_t converged;
+    int32_t status;         /* FP_OK or FP_ERR_NONFINITE                      */
+    double  fieldres;       /* 2D only: field residual (LMB2d/lbm.cpp:169-182) */
+} fp_solve_out;
+
+/* Everything lbm3d-perm writes to its CSV row (LMB3d/lbm3d-perm.cpp:574-615)
+ * plus the --validate tensor test (:555-572). */
+typedef struct fp_result3d {
+    int32_t nx, ny, nz, pad0;
+    int64_t n_pore;
+    double  porosity, force, mu;
+    fp_solve_out run[3];    /* forcing along +x, +y, +z                       */
+    double  K[3][3];        /* K[flux component][force axis]                  */
+    double  k_xy, k_xz, k_yz;
+    double  recip_xy, recip_xz, recip_yz, recip_resid;
+    double  k[3];           /* principal values, descending                   */
+    double  k_mean, fa;
+    double  v[3][3];        /* v[component][eigen index] (cyclic Jacobi, :401-443) */
+    int32_t validated, pad1;
+    fp_solve_out val;       /* the (1,1,1) solve                              */
+    double  val_pred[3];
+    double  val_err;
+    double  seconds;        /* wall time of the solves                        */
+    int64_t fluid_updates;  /* fluid-cell updates performed                   */
+} fp_result3d;
+
+/* Check a mask as read_structure does (LMB3d/lbm3d-perm.cpp:182-192):
+ * FP_OK, FP_ERR_BAD_DIMS, FP_ERR_NON_BINARY or FP_ERR_NO_PORE.  n_pore may be
+ * NULL. */
+int fp_check_mask3d(const uint8_t *solid, int nx, int ny, int nz, int64_t *n_pore);
+
+/* The whole lbm3d-perm invocation minus file I/O (LMB3d/lbm3d-perm.cpp:507-630):
+ * three solves along +x, +y, +z (+ the tensor test when cfg->validate), K,
+ * reciprocity, eigen-decomposition and FA.  vel, when not NULL, receives the
+ * export_velocity payload of each direction (:305-317): 3 runs x NN x 3
+ * floats (U, V, W per node, x fastest), i.e. what PREFIX.f{x,y,z}.vel hold
+ * after their 16-byte header.  Returns FP_OK, FP_NOT_CONVERGED or an error. */
+int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
+               const fp_config3d *cfg, fp_result3d *out, float *vel);
+
+/* One forcing direction: solve() of LMB3d/lbm3d-perm.cpp:320-398 for the body
+ * force F (lattice units).  vel: NULL or NN*3 floats. */
+int fp_solve_dir3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
+                   const fp_config3d *cfg, const double F[3], fp_solve_out *out, float *vel);
+
+/* Symmetric 3x3 eigen-decomposition exactly as jacobi3 (:401-443). */
+void fp_jacobi3(const double A[9], double val[3], double vec[9]);
+
+/* ------------------------------------------------------------------------ */
+/* 2D: D2Q9                                                                  */
+/* ------------------------------------------------------------------------ */
+/* Config + flags of LMB2d/main.cpp:64-79 and :286-308; L0 of LMB2d/lbm.h:27. */
+typedef struct fp_config2d {
+    double  ff;             /* (1)       */
+    double  eps;            /* (1e-3)    */
+    double  qfloor;         /* (1e-2)    */
+    double  ksigma;         /* (3)       */
+    double  L0;             /* flux normalisation length (4 at 256x256; the
+                               reference scales it with the lattice, lbm.h:24-27) */
+    double  validate_deg;   /* --validate DEG; < 0 = off                    */
+    int64_t lag;            /* (5000)    */
+    int64_t min_steps;      /* (10000)   */
+    int64_t max_steps;      /* (500000)  */
+    int64_t avg_steps;      /* (20000)   */
+    int64_t avg_sample;     /* (50)      */
+    int32_t hold;           /* (3)       */
+    int32_t precision;      /* enum fp_precision (FP_F32_REF2D = the reference) */
+    int32_t concurrent;
+    int32_t pad;
+} fp_config2d;
+
+void fp_config2d_default(fp_config2d *cfg);
+
+/* Everything lbm2d-perm writes to its CSV row (LMB2d/main.cpp:382-411). */
+typedef struct fp_result2d {
+    int32_t lx, ly;
+    double  porosity, force, mu;
+    fp_solve_out run[2];    /* forcing along +x, +y (q[2], se[2] unused)     */
+    double  k_xx, k_yx, k_xy, k_yy, k_off, recip_resid;
+    double  k1, k2, k_mean, anisotropy, theta_deg, alpha_x_deg, alpha_y_deg;
+    int32_t validated, pad1;
+    fp_solve_out val;
+    double  val_pred[2];
+    double  val_err;
+    double  seconds;
+    int64_t fluid_updates;
+} fp_result2d;
+
+/* The whole lbm2d-perm invocation minus file I/O (LMB2d/main.cpp:310-426).
+ * solid is [ly][lx].  vel: NULL or 2 runs x (ly*lx) x 2 floats ([y][x] rows
+ * of (U, V), what exportvelocity, LMB2d/lbm.cpp:587-594, prints). */
+int fp_solve2d(fp_ctx *ctx, const uint8_t *solid, int lx, int ly,
+               const fp_config2d *cfg, fp_result2d *out, float *vel);
+
+/* One forcing direction: solve() of LMB2d/main.cpp:99-235. */
+int fp_solve_dir2d(fp_ctx *ctx, const uint8_t *solid, int lx, int ly,
+                   const fp_config2d *cfg, const double F[2], fp_solve_out *out, float *vel);
+
+/* ------------------------------------------------------------------------ */
+/* batched solving (no reference counterpart: the reference runs one process
+ * per structure, scripts/solve_permeability3d.py:250).  Every job of a batch
+ * is one forced solve; all live jobs advance together, each retires on the
+ * device when its own convergence test passes.                              */
+/* ------------------------------------------------------------------------ */
+typedef struct fp_batch fp_batch;
+
+/* lattice: 3 (D3Q19) or 2 (D2Q9); precision: enum fp_precision */
+fp_batch *fp_batch_create(fp_ctx *ctx, int lattice, int precision, int *status);
+void      fp_batch_destroy(fp_batch *b);
+/* add one forced solve of a 3D mask with cfg's convergence settings; returns
+ * a job id >= 0, or -status.  Masks with identical bytes share one device
+ * geometry. */
+int64_t   fp_batch_add3d(fp_batch *b, const uint8_t *solid, int nx, int ny, int nz,
+                         const fp_config3d *cfg, const double F[3]);
+int64_t   fp_batch_add2d(fp_batch *b, const uint8_t *solid, int lx, int ly,
+                         const fp_config2d *cfg, const double F[2]);
+/* run every job to completion */
+int       fp_batch_run(fp_batch *b);
+int       fp_batch_result(fp_batch *b, int64_t id, fp_solve_out *out);
+/* steps launched, lattice-cell updates performed, seconds on the device */
+void      fp_batch_stats(fp_batch *b, int64_t *steps, int64_t *fluid_updates, double *seconds);
+/* steps per hipGraph chunk (even; default 200) */
+void      fp_batch_set_chunk(fp_batch *b, int steps);
+
+/* ------------------------------------------------------------------------ */
+/* measurement (no reference counterpart: the reference timed its solver by
+ * hand, NOTES.md:1193-1205).  Steps one lattice at rest under force F for
+ * `warmup` then `steps` steps (even counts), with a HIP event pair around
+ * every kernel on the context's stream.                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct fp_bench_out {
+    int64_t n_fluid;          /* fluid cells (one update each per step)     */
+    int64_t n_cells;          /* all lattice cells                          */
+    int64_t steps;            /* timed steps                                */
+    double  total_ms;         /* event time of the whole timed region       */
+    double  local_ms;         /* average duration: local (odd) step kernel  */
+    double  nbr_ms;           /* average duration: neighbour (even) kernel  */
+    double  monitor_ms;       /* average duration: convergence monitor      */
+    double  bytes_per_update; /* algorithmic bytes per fluid update: 2 Q s  */
+} fp_bench_out;
+
+int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
+                   int precision, const double F[3], int warmup, int steps, fp_bench_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLOWPERM_H */

@@ -1,0 +1,218 @@
+"""
+ctypes binding of oracle/liboracle.so -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker (or the timed CPU baseline).  The product
+(flow-anizotropy_amd/flowperm) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_BIN = HERE / "_ref" / "lbm3d-perm"
+
+
+class Cfg3(C.Structure):
+    _fields_ = [("ff", C.c_double), ("eps", C.c_double), ("lag", C.c_long), ("hold", C.c_int),
+                ("min_steps", C.c_long), ("max_steps", C.c_long), ("avg_steps", C.c_long)]
+
+
+class Solve3(C.Structure):
+    _fields_ = [("q", C.c_double * 3), ("se", C.c_double * 3), ("steps", C.c_long),
+                ("converged", C.c_int), ("status", C.c_int), ("fail_step", C.c_long)]
+
+
+class Tensor3(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("n_pore", C.c_long),
+                ("porosity", C.c_double), ("force", C.c_double), ("mu", C.c_double),
+                ("run", Solve3 * 3), ("K", C.c_double * 9),
+                ("k_xy", C.c_double), ("k_xz", C.c_double), ("k_yz", C.c_double),
+                ("recip_xy", C.c_double), ("recip_xz", C.c_double), ("recip_yz", C.c_double),
+                ("recip_resid", C.c_double), ("kval", C.c_double * 3), ("k_mean", C.c_double),
+                ("fa", C.c_double), ("vec", C.c_double * 9), ("validated", C.c_int),
+                ("val", Solve3), ("val_pred", C.c_double * 3), ("val_err", C.c_double),
+                ("status", C.c_int)]
+
+
+class Cfg2(C.Structure):
+    _fields_ = [("ff", C.c_double), ("eps", C.c_double), ("lag", C.c_long), ("hold", C.c_int),
+                ("min_steps", C.c_long), ("max_steps", C.c_long), ("avg_steps", C.c_long),
+                ("avg_sample", C.c_long), ("qfloor", C.c_double), ("ksigma", C.c_double),
+                ("L0", C.c_double)]
+
+
+class Solve2(C.Structure):
+    _fields_ = [("qx", C.c_double), ("qy", C.c_double), ("se_qx", C.c_double),
+                ("se_qy", C.c_double), ("steps", C.c_long), ("converged", C.c_int),
+                ("fieldres", C.c_double), ("status", C.c_int), ("fail_step", C.c_long)]
+
+
+class Tensor2(C.Structure):
+    _fields_ = [("LX", C.c_int), ("LY", C.c_int), ("porosity", C.c_double), ("force", C.c_double),
+                ("mu", C.c_double), ("run", Solve2 * 2),
+                ("k_xx", C.c_double), ("k_yx", C.c_double), ("k_xy", C.c_double),
+                ("k_yy", C.c_double), ("k_off", C.c_double), ("recip", C.c_double),
+                ("k1", C.c_double), ("k2", C.c_double), ("k_mean", C.c_double),
+                ("aniso", C.c_double), ("theta", C.c_double), ("alpha_x", C.c_double),
+                ("alpha_y", C.c_double), ("validated", C.c_int), ("val", Solve2),
+                ("val_pred", C.c_double * 2), ("val_err", C.c_double), ("status", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        _lib = C.CDLL(str(LIB))
+        _lib.or3_solve_dir.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Cfg3),
+                                       C.c_double, C.c_double, C.c_double, C.POINTER(Solve3),
+                                       C.c_void_p]
+        _lib.or3_solve_tensor.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Cfg3),
+                                          C.c_int, C.POINTER(Tensor3)]
+        _lib.or3_run_steps.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_long, C.c_void_p, C.c_void_p]
+        _lib.or3_mu.restype = C.c_double
+        _lib.or_fix96_sum.argtypes = [C.c_void_p, C.c_long, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+        _lib.or3_jacobi3.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or3_check_mask.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_long)]
+        for pfx in ("or2_", "or2d_"):          # float (reference) and double variants
+            getattr(_lib, pfx + "solve_dir").argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Cfg2),
+                                                         C.c_double, C.c_double, C.POINTER(Solve2),
+                                                         C.c_void_p]
+            getattr(_lib, pfx + "solve_tensor").argtypes = [C.c_void_p, C.c_int, C.c_int,
+                                                            C.POINTER(Cfg2), C.c_double,
+                                                            C.POINTER(Tensor2)]
+            getattr(_lib, pfx + "run_steps").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                                         C.c_double, C.c_long, C.c_void_p, C.c_void_p]
+            getattr(_lib, pfx + "mu").restype = C.c_float
+    return _lib
+
+
+def cfg3(ff=1.0, eps=1e-5, lag=2000, hold=2, min_steps=4000, max_steps=200000, avg_steps=2000):
+    return Cfg3(float(ff), float(eps), int(lag), int(hold), int(min_steps), int(max_steps),
+                int(avg_steps))
+
+
+def cfg2(ff=1.0, eps=1e-3, lag=5000, hold=3, min_steps=10000, max_steps=500000,
+         avg_steps=20000, avg_sample=50, qfloor=1e-2, ksigma=3.0, L0=4.0):
+    return Cfg2(float(ff), float(eps), int(lag), int(hold), int(min_steps), int(max_steps),
+                int(avg_steps), int(avg_sample), float(qfloor), float(ksigma), float(L0))
+
+
+def _u8(m):
+    m = np.ascontiguousarray(m, dtype=np.uint8)
+    return m, m.ctypes.data_as(C.c_void_p)
+
+
+def solve_dir3d(mask, F, velocity=False, **cfg):
+    m, p = _u8(mask)
+    nz, ny, nx = m.shape
+    out = Solve3()
+    vel = np.zeros((nz, ny, nx, 3), np.float32) if velocity else None
+    lib().or3_solve_dir(p, nx, ny, nz, C.byref(cfg3(**cfg)), F[0], F[1], F[2], C.byref(out),
+                        vel.ctypes.data_as(C.c_void_p) if velocity else None)
+    r = {"q": np.array(out.q[:]), "se": np.array(out.se[:]), "steps": out.steps,
+         "converged": out.converged, "status": out.status}
+    if velocity:
+        r["velocity"] = vel
+    return r
+
+
+def solve_tensor3d(mask, validate=False, **cfg):
+    m, p = _u8(mask)
+    nz, ny, nx = m.shape
+    T = Tensor3()
+    rc = lib().or3_solve_tensor(p, nx, ny, nz, C.byref(cfg3(**cfg)), 1 if validate else 0,
+                                C.byref(T))
+    return rc, T
+
+
+def run_steps3d(mask, F, steps):
+    m, p = _u8(mask)
+    nz, ny, nx = m.shape
+    f = np.zeros((nz * ny * nx, 19), np.float64)
+    uvw = np.zeros((nz * ny * nx, 3), np.float64)
+    lib().or3_run_steps(p, nx, ny, nz, F[0], F[1], F[2], steps, f.ctypes.data_as(C.c_void_p),
+                        uvw.ctypes.data_as(C.c_void_p))
+    return f, uvw
+
+
+def check_mask3d(mask):
+    """read_structure's checks: (code, n_pore); code 0 ok, 1 dims, 2 non-binary, 3 no pore."""
+    m, p = _u8(mask)
+    nz, ny, nx = m.shape
+    n = C.c_long(0)
+    return lib().or3_check_mask(p, nx, ny, nz, C.byref(n)), n.value
+
+
+def fix96_sum(values):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    hi, lo, d = C.c_int64(), C.c_uint64(), C.c_double()
+    bad = lib().or_fix96_sum(v.ctypes.data_as(C.c_void_p), v.size, C.byref(hi), C.byref(lo),
+                             C.byref(d))
+    return (hi.value << 64) + lo.value, d.value, bad
+
+
+def jacobi3(A):
+    a = np.ascontiguousarray(np.asarray(A, float).reshape(9))
+    val = np.zeros(3)
+    vec = np.zeros(9)
+    lib().or3_jacobi3(a.ctypes.data_as(C.c_void_p), val.ctypes.data_as(C.c_void_p),
+                      vec.ctypes.data_as(C.c_void_p))
+    return val, vec.reshape(3, 3)
+
+
+def _p2(double):
+    return "or2d_" if double else "or2_"
+
+
+def solve_dir2d(mask, F, velocity=False, double=False, **cfg):
+    """D2Q9 solve(): float = the reference arithmetic; double=True = same scheme in double."""
+    m, p = _u8(mask)
+    ly, lx = m.shape
+    out = Solve2()
+    vel = np.zeros((ly, lx, 2), np.float32) if velocity else None
+    getattr(lib(), _p2(double) + "solve_dir")(p, lx, ly, C.byref(cfg2(**cfg)), F[0], F[1], C.byref(out),
+                        vel.ctypes.data_as(C.c_void_p) if velocity else None)
+    r = {"q": np.array([out.qx, out.qy]), "se": np.array([out.se_qx, out.se_qy]),
+         "steps": out.steps, "converged": out.converged, "fieldres": out.fieldres,
+         "status": out.status}
+    if velocity:
+        r["velocity"] = vel
+    return r
+
+
+def solve_tensor2d(mask, validate=-1.0, double=False, **cfg):
+    m, p = _u8(mask)
+    ly, lx = m.shape
+    T = Tensor2()
+    rc = getattr(lib(), _p2(double) + "solve_tensor")(p, lx, ly, C.byref(cfg2(**cfg)), float(validate), C.byref(T))
+    return rc, T
+
+
+def run_steps2d(mask, F, steps, double=False):
+    m, p = _u8(mask)
+    ly, lx = m.shape
+    df = np.zeros((lx, ly, 9), np.float64 if double else np.float32)
+    uv = np.zeros((ly, lx, 2), np.float32)
+    getattr(lib(), _p2(double) + "run_steps")(p, lx, ly, F[0], F[1], steps, df.ctypes.data_as(C.c_void_p),
+                        uv.ctypes.data_as(C.c_void_p))
+    return df, uv
+
+
+def run_reference(raw_path, csv_path, flags=(), cwd=None, timeout=None):
+    """Run the reference binary built by oracle/Makefile (oracle/_ref/lbm3d-perm)."""
+    if not REF_BIN.exists():
+        raise RuntimeError(f"{REF_BIN} missing: run `make -C oracle ref` where the reference is mounted")
+    return subprocess.run([str(REF_BIN), str(raw_path), str(csv_path)] + list(flags),
+                          capture_output=True, text=True, cwd=cwd, timeout=timeout)

@@ -1,0 +1,80 @@
+import json
+import math
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+REPO = Path(__file__).resolve().parent.parent
+GOLDEN = REPO / "tests" / "golden"
+sys.path.insert(0, str(REPO / "flow-anizotropy_amd"))
+sys.path.insert(0, str(REPO))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+def golden(name):
+    """(mask [z,y,x] uint8, reference CSV row as dict of strings/floats, meta dict)."""
+    from flowperm import structio
+    import pandas as pd
+    mask = structio.load_raw(GOLDEN / f"{name}.raw", check=False)
+    row = pd.read_csv(GOLDEN / f"{name}.csv").iloc[0].to_dict()
+    meta = json.loads((GOLDEN / f"{name}.json").read_text())
+    return np.array(mask), row, meta
+
+
+def golden_flags(meta):
+    """Reference argv flags -> keyword config."""
+    argv = meta["argv"][3:]
+    cfg = {}
+    names = {"--ff": ("ff", float), "--eps": ("eps", float), "--lag": ("lag", int),
+             "--hold": ("hold", int), "--min-steps": ("min_steps", int),
+             "--max-steps": ("max_steps", int), "--avg-steps": ("avg_steps", int)}
+    i = 0
+    while i < len(argv):
+        a = argv[i]
+        if a in names:
+            k, t = names[a]
+            cfg[k] = t(argv[i + 1])
+            i += 2
+        else:
+            i += 1
+    return cfg
+
+
+def printed_tol(ref):
+    """Half a unit in the 10th significant digit: the precision of a value the
+    reference printed with setprecision(10)."""
+    if ref == 0 or not math.isfinite(ref):
+        return 0.0
+    e = math.floor(math.log10(abs(ref)))
+    return 0.5 * 10.0 ** (e - 9) * 1.000001
+
+
+def assert_printed_close(ours, ref, floor=0.0, what=""):
+    tol = max(printed_tol(ref), floor)
+    assert abs(ours - ref) <= tol, f"{what}: ours {ours!r} ref {ref!r} |d|={abs(ours-ref):.3e} tol {tol:.3e}"
+
+
+GOLDEN_FAST = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5"]
+GOLDEN_ALL = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5", "sph_x28y24z20_s1",
+              "slit32_h10", "slit32_h20", "c1_64_s0", "c1_64_s1", "ell24_validate_s3"]
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import binding
+    binding.lib()
+    return binding
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    from flowperm import solver
+    ctx = solver.Context()
+    yield ctx
+    ctx.close()

@@ -1,0 +1,183 @@
+"""
+D3Q19 on the GPU against the oracle and the reference binary's rows.
+
+FP64 is bit-for-bit the oracle (same operations, exact flux sums): every q,
+se, K, step count and convergence flag compares with ==.  Against the
+reference's own CSV rows the bar is the printed precision (10 significant
+digits) with a 1e-10 * trace floor, and identical step counts.  FP32 (shifted
+populations, FP64 sums) must stay within 1e-4 of FP64 per component, relative
+to the component or, for near-zero off-diagonals, to the trace.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_ALL, GOLDEN_FAST, assert_printed_close, golden, golden_flags
+
+pytestmark = pytest.mark.gpu
+
+TAGS = ("fx", "fy", "fz")
+AX = "xyz"
+
+
+def gpu_row(ctx, mask, **cfg):
+    from flowperm import solver
+    return solver.solve_permeability3d(mask, ctx=ctx, **cfg)
+
+
+@pytest.mark.parametrize("name", GOLDEN_FAST + ["sph_x28y24z20_s1", "ell24_validate_s3"])
+def test_fp64_bit_exact_against_oracle(gpu_ctx, oracle, name):
+    mask, _, meta = golden(name)
+    cfg = golden_flags(meta)
+    print(f"case {name} shape {mask.shape} cfg {cfg}", flush=True)
+    rc, T = oracle.solve_tensor3d(mask, **cfg)
+    r = gpu_row(gpu_ctx, mask, **cfg)
+    assert r["exit_code"] == rc
+    for d, t in enumerate(TAGS):
+        assert r[f"steps_{t}"] == T.run[d].steps
+        assert r[f"conv_{t}"] == T.run[d].converged
+        for k, a in enumerate(AX):
+            assert r[f"q{a}_{t}"] == T.run[d].q[k], (t, a)
+            assert r[f"se_q{a}_{t}"] == T.run[d].se[k], (t, a)
+    K = np.array(T.K[:]).reshape(3, 3)
+    for i, a in enumerate(AX):
+        for j, b in enumerate(AX):
+            assert r[f"K_{a}{b}"] == K[i, j]
+    assert r["k1"] == T.kval[0] and r["k3"] == T.kval[2] and r["fa"] == T.fa
+    assert r["recip_resid"] == T.recip_resid
+
+
+@pytest.mark.parametrize("name", GOLDEN_ALL)
+def test_fp64_matches_reference_rows(gpu_ctx, name):
+    mask, row, meta = golden(name)
+    cfg = golden_flags(meta)
+    print(f"case {name} shape {mask.shape}", flush=True)
+    r = gpu_row(gpu_ctx, mask, validate="--validate" in meta["argv"], **cfg)
+    assert r["exit_code"] == meta["returncode"]
+    K = np.array([[r[f"K_{a}{b}"] for b in AX] for a in AX])
+    trace = abs(np.trace(K))
+    for t in TAGS:
+        assert r[f"steps_{t}"] == row[f"steps_{t}"], t
+        assert r[f"conv_{t}"] == row[f"conv_{t}"], t
+        qn = float(np.linalg.norm([r[f"q{a}_{t}"] for a in AX]))
+        for a in AX:
+            assert_printed_close(r[f"q{a}_{t}"], row[f"q{a}_{t}"], 1e-10 * qn, f"q{a}_{t}")
+            ref_se = row[f"se_q{a}_{t}"]
+            assert abs(r[f"se_q{a}_{t}"] - ref_se) <= max(3e-8 * qn, 1e-3 * abs(ref_se))
+    for a in AX:
+        for b in AX:
+            assert_printed_close(r[f"K_{a}{b}"], row[f"K_{a}{b}"], 1e-10 * trace, f"K_{a}{b}")
+    for key in ("k_xy", "k_xz", "k_yz", "k1", "k2", "k3", "k_mean"):
+        assert_printed_close(r[key], row[key], 1e-10 * trace, key)
+    assert_printed_close(r["porosity"], row["porosity"], 0, "porosity")
+    if "--validate" in meta["argv"]:
+        # the binary prints the tensor test on stdout (:566-571)
+        line = [l for l in meta["stdout"] if "relative error" in l][0]
+        ref_err = float(line.split()[-1])
+        assert abs(r["val_err"] - ref_err) <= 1e-5 * ref_err + 1e-12
+
+
+def test_fp64_concurrent_directions_bitwise_equal(gpu_ctx):
+    mask, _, meta = golden("sph_x28y24z20_s1")
+    cfg = golden_flags(meta)
+    a = gpu_row(gpu_ctx, mask, **cfg)
+    b = gpu_row(gpu_ctx, mask, concurrent=True, **cfg)
+    for key in [c for c in a if c.startswith(("K_", "q", "se_", "steps_", "conv_"))]:
+        assert a[key] == b[key], key
+
+
+@pytest.mark.parametrize("h", [6, 10])
+def test_slit_closed_form(gpu_ctx, h):
+    from flowperm import masks
+    n = 16
+    r = gpu_row(gpu_ctx, masks.slit(n, h), eps=1e-10, lag=500, min_steps=2000,
+                max_steps=20000, avg_steps=200)
+    exact = h * (h * h + 1) / (12.0 * n)
+    assert abs(r["K_xx"] - exact) / exact < 1e-9
+    assert abs(r["K_yy"] - exact) / exact < 1e-9
+    assert abs(r["K_zz"]) < 1e-9 * exact
+
+
+@pytest.mark.parametrize("name", ["sph16_s0", "ell_x24y20z16_fast_s5", "c1_64_s0"])
+def test_fp32_within_1e4_of_fp64(gpu_ctx, name):
+    mask, _, meta = golden(name)
+    cfg = golden_flags(meta)
+    a = gpu_row(gpu_ctx, mask, **cfg)
+    b = gpu_row(gpu_ctx, mask, precision="f32", **cfg)
+    K64 = np.array([[a[f"K_{x}{y}"] for y in AX] for x in AX])
+    K32 = np.array([[b[f"K_{x}{y}"] for y in AX] for x in AX])
+    trace = abs(np.trace(K64))
+    err = np.abs(K32 - K64) / np.maximum(np.abs(K64), trace)
+    print(f"{name}: max FP32 error per component (floor trace) {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-4
+
+
+def test_velocity_export_matches_oracle_and_reference(gpu_ctx, oracle):
+    """export_velocity (:305-317): the .vel payload of each direction."""
+    from conftest import GOLDEN
+    name = "sph12_velocity_s4"
+    mask, row, meta = golden(name)
+    cfg = golden_flags(meta)
+    r = gpu_row(gpu_ctx, mask, velocity=True, **cfg)
+    vel = r["velocity"]
+    nz, ny, nx = mask.shape
+    F = 2.5e-7 * cfg.get("ff", 1.0)
+    for d, t in enumerate(TAGS):
+        Fv = [0.0, 0.0, 0.0]
+        Fv[d] = F
+        o = oracle.solve_dir3d(mask, Fv, velocity=True, **cfg)
+        assert np.array_equal(vel[d], o["velocity"]), t
+        raw = (GOLDEN / f"{name}.{t}.vel").read_bytes()
+        hdr = np.frombuffer(raw, "<i4", count=4)
+        assert list(hdr) == [nx, ny, nz, 3]
+        ref = np.frombuffer(raw, "<f4", offset=16).reshape(nz, ny, nx, 3)
+        scale = np.abs(ref).max()
+        assert np.abs(vel[d] - ref).max() <= 1e-6 * scale, t
+        assert np.all(vel[d][mask == 1] == 0)           # mask-zero
+
+
+def test_mask_errors_map_to_reference_exit_codes(gpu_ctx):
+    from flowperm import solver
+    from flowperm._lib import FlowpermError
+    m = np.zeros((4, 4, 4), np.uint8)
+    m[0] = 1
+    bad = m.copy()
+    bad[1, 1, 1] = 2
+    for arr, status in ((bad, 11), (np.ones((4, 4, 4), np.uint8), 12)):
+        with pytest.raises(FlowpermError) as e:
+            solver.solve_permeability3d(arr, ctx=gpu_ctx, max_steps=10, avg_steps=0)
+        assert e.value.status == status
+
+
+def test_runaway_lattice_is_a_nonfinite_flux(gpu_ctx):
+    """|u| >= 1 or NaN makes the flux non-finite: status FP_ERR_NONFINITE (exit 2)."""
+    from flowperm import masks, solver
+    from flowperm._lib import FlowpermError
+    with pytest.raises(FlowpermError) as e:
+        solver.solve_permeability3d(masks.slit(8, 4), ctx=gpu_ctx, ff=1e9, lag=10,
+                                    max_steps=200, avg_steps=0)
+    assert e.value.status == 13
+
+
+def test_max_steps_without_convergence_is_exit_3(gpu_ctx, oracle):
+    mask, _, _ = golden("sph16_s0")
+    cfg = dict(eps=1e-30, lag=100, min_steps=100, max_steps=600, avg_steps=50)
+    r = gpu_row(gpu_ctx, mask, **cfg)
+    rc, T = oracle.solve_tensor3d(mask, **cfg)
+    assert r["exit_code"] == 3 == rc
+    assert r["steps_fx"] == T.run[0].steps == 650
+    assert r["qx_fx"] == T.run[0].q[0]
+
+
+@pytest.mark.parametrize("max_steps,avg_steps,lag", [(0, 0, 7), (0, 5, 7), (9, 0, 4), (8, 0, 4),
+                                                     (13, 3, 5), (1, 1, 1)])
+def test_step_edge_cases_bit_exact(gpu_ctx, oracle, max_steps, avg_steps, lag):
+    """odd/even step counts, no phase 1, no averaging: the step semantics of
+    solve() (:332-392) at its edges."""
+    mask, _, _ = golden("sph16_s0")
+    cfg = dict(eps=1e-5, lag=lag, min_steps=0, max_steps=max_steps, avg_steps=avg_steps)
+    for F in ([2.5e-7, 0, 0], [0, 0, 2.5e-7]):
+        o = oracle.solve_dir3d(mask, F, **cfg)
+        from flowperm import solver
+        g = solver.solve_direction3d(mask, F, ctx=gpu_ctx, **cfg)
+        assert g["steps"] == o["steps"] and g["converged"] == o["converged"]
+        assert np.array_equal(g["q"], o["q"]) and np.array_equal(g["se"], o["se"])
