@@ -37,8 +37,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload_mask(n, seed):
+def workload_mask(n, seed, kind="ellipsoids"):
     from flowperm import masks
+    if kind == "open":                       # diagnostic: every cell fluid
+        log(f"mask {n}^3 open (all fluid)")
+        return np.zeros((n, n, n), np.uint8), 1.0
     rng = np.random.default_rng(seed)
     por = 0.3 + 0.5 * rng.random()
     t0 = time.time()
@@ -49,13 +52,17 @@ def workload_mask(n, seed):
     return m, por
 
 
-def bench_steps(ctx, mask, prec, warmup, steps):
+def bench_steps(ctx, mask, prec, warmup, steps, tune=None):
     from flowperm import _lib
     nz, ny, nx = mask.shape
     out = _lib.BenchOut()
     F = np.array([2.5e-7 * 40, 0.0, 0.0])
+    t = None
+    if tune is not None:
+        t = _lib.Tuning(tune[0], tune[1], 200, tune[2], tune[3], tune[4])
     st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, 3, prec,
-                                _lib.f64ptr(F), warmup, steps, C.byref(out))
+                                _lib.f64ptr(F), warmup, steps, C.byref(t) if t else None,
+                                C.byref(out))
     if st != 0:
         raise RuntimeError(f"fp_bench_steps failed: {st} {ctx.last_error()}")
     return out
@@ -110,8 +117,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--mask", default="ellipsoids", choices=["ellipsoids", "open"],
+                    help="workload mask (open = all fluid, a diagnostic)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
+    ap.add_argument("--tune", default="", help="fixed launch shapes 'cpt,waves,nt,local_bpc,nbr_bpc' "
+                    "for both precisions instead of running the autotuner")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -120,14 +131,32 @@ def main():
         sys.exit(2)
 
     from flowperm import solver
-    mask, por_target = workload_mask(a.size, a.seed)
+    mask, por_target = workload_mask(a.size, a.seed, a.mask)
     n_fluid = int(mask.size - np.count_nonzero(mask))
     ctx = solver.Context()
     steps = max(2, a.steps & ~1)
     warm = max(0, a.warmup & ~1)
 
+    from flowperm import autotune
+    tuned = {}
+    for prec, name in ((1, "f32"), (0, "f64")):
+        if prec == 0 and a.no_fp64:
+            continue
+        if a.tune:
+            c, w, n, lb, nb = (int(v) for v in a.tune.split(","))
+            tuned[name] = {"local_cpt": c, "nbr_waves": w, "nontemporal": n, "local_bpc": lb,
+                           "nbr_bpc": nb, "chunk_steps": 200, "fixed": True}
+            continue
+        rec = autotune.tune(ctx, mask, 3, prec, steps=20, use_cache=False)
+        tuned[name] = rec
+        log(f"autotune {name}: best {({k: rec[k] for k in autotune.FIELDS})}")
+        log(f"  local ms {rec['local_ms']}")
+        log(f"  neighbour ms {rec['nbr_ms']}")
+    def shape(r):
+        return (r["local_cpt"], r["nbr_waves"], r["nontemporal"], r["local_bpc"], r["nbr_bpc"])
+    t32 = shape(tuned["f32"])
     log(f"FP32 D3Q19 {a.size}^3: warmup {warm}, timed {steps} steps")
-    o32 = bench_steps(ctx, mask, 1, warm, steps)
+    o32 = bench_steps(ctx, mask, 1, warm, steps, t32)
     ms_step = o32.total_ms / o32.steps
     mlups = o32.n_fluid * o32.steps / (o32.total_ms / 1e3) / 1e6
     kern_ms = 0.5 * (o32.local_ms + o32.nbr_ms)
@@ -138,9 +167,15 @@ def main():
         f"step kernels {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.3f} of peak")
 
     extra = {}
+    copy_ms = C.c_double(0)
+    copy_bytes = 2 * 1024 ** 3
+    if ctx.lib.fp_bench_copy(ctx.handle, copy_bytes, 20, C.byref(copy_ms)) == 0:
+        extra["copy_GBs"] = round(2 * copy_bytes / (copy_ms.value / 1e3) / 1e9, 1)
+        log(f"plain copy of {copy_bytes >> 20} MiB: {extra['copy_GBs']:.0f} GB/s (read + write)")
     if not a.no_fp64:
         s64 = max(2, (steps // 2) & ~1)
-        o64 = bench_steps(ctx, mask, 0, max(2, warm // 2), s64)
+        t64 = shape(tuned["f64"])
+        o64 = bench_steps(ctx, mask, 0, max(2, warm // 2), s64, t64)
         k64 = 0.5 * (o64.local_ms + o64.nbr_ms)
         extra["fp64"] = {
             "mlups": o64.n_fluid * o64.steps / (o64.total_ms / 1e3) / 1e6,
@@ -185,6 +220,7 @@ def main():
                       "over the mean event-timed duration of the two step kernels",
             "local_ms": round(o32.local_ms, 5), "neighbour_ms": round(o32.nbr_ms, 5),
             "monitor_ms": round(o32.monitor_ms, 5)},
+        "autotune": tuned,
         "cpu_baseline": cpu,
     }
     line.update(extra)

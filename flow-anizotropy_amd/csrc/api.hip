@@ -17,6 +17,8 @@ struct fp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    Tuning tune;
+    int chunk = 200;
 };
 
 namespace {
@@ -145,7 +147,8 @@ std::string run_jobs(fp_ctx *ctx, std::vector<Job *> &jobs, int lattice, int pre
                                                    std::max<int64_t>(0, j->cfg.avg_steps));
     const int chunk = (int)std::min<int64_t>(200, std::max<int64_t>(2, (maxsteps + 1) & ~1ll));
     auto step_until_done = [&](const std::vector<Job *> &set) -> std::string {
-        Batch b(lattice, prec, ctx->stream, chunk, true);
+        Batch b(lattice, prec, ctx->stream, std::min(chunk, ctx->chunk), true);
+        b.set_tuning(ctx->tune);
         std::string err;
         for(Job *j : set) {
             err = b.admit(j);
@@ -650,7 +653,8 @@ fp_batch *fp_batch_create(fp_ctx *ctx, int lattice, int precision, int *status)
     B->ctx = ctx;
     B->lattice = lattice;
     B->prec = precision;
-    B->b.reset(new Batch(lattice, precision, ctx->stream, 200, true));
+    B->b.reset(new Batch(lattice, precision, ctx->stream, ctx->chunk, true));
+    B->b->set_tuning(ctx->tune);
     if(status) *status = FP_OK;
     return B;
 }
@@ -761,10 +765,42 @@ void fp_batch_set_chunk(fp_batch *B, int steps)
 // ---------------------------------------------------------------------------
 // benchmark entry: time stepping of one resident lattice
 // ---------------------------------------------------------------------------
+static bool tuning_ok(const fp_tuning *t)
+{
+    return t && (t->local_cpt == 1 || t->local_cpt == 2 || t->local_cpt == 4) &&
+           t->nbr_waves >= 1 && t->nbr_waves <= 8 && t->chunk_steps >= 2 &&
+           (t->nontemporal == 0 || t->nontemporal == 1) &&
+           t->local_bpc >= 0 && t->local_bpc <= 32 && t->nbr_bpc >= 0 && t->nbr_bpc <= 32;
+}
+
+extern "C" int fp_set_tuning(fp_ctx *ctx, const fp_tuning *t)
+{
+    if(!ctx || !tuning_ok(t)) return FP_ERR_ARG;
+    ctx->tune.local_cpt = t->local_cpt;
+    ctx->tune.nbr_waves = t->nbr_waves;
+    ctx->tune.nontemporal = t->nontemporal;
+    ctx->tune.local_bpc = t->local_bpc;
+    ctx->tune.nbr_bpc = t->nbr_bpc;
+    ctx->chunk = t->chunk_steps & ~1;
+    return FP_OK;
+}
+
+extern "C" void fp_get_tuning(fp_ctx *ctx, fp_tuning *t)
+{
+    if(!ctx || !t) return;
+    t->local_cpt = ctx->tune.local_cpt;
+    t->nbr_waves = ctx->tune.nbr_waves;
+    t->chunk_steps = ctx->chunk;
+    t->nontemporal = ctx->tune.nontemporal;
+    t->local_bpc = ctx->tune.local_bpc;
+    t->nbr_bpc = ctx->tune.nbr_bpc;
+}
+
 extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
                               int precision, const double F[3], int warmup, int steps,
-                              fp_bench_out *out)
+                              const fp_tuning *tune, fp_bench_out *out)
 {
+    if(tune && !tuning_ok(tune)) return FP_ERR_ARG;
     if(!ctx || !solid || !out || (lattice != 2 && lattice != 3) || steps < 2 || warmup < 0)
         return FP_ERR_ARG;
     std::memset(out, 0, sizeof(*out));
@@ -791,6 +827,12 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
         j.cfg = cfg2(c, nx, ny);
     }
     Batch b(lattice, precision, ctx->stream, 2, false);
+    Tuning tu = ctx->tune;
+    if(tune) {
+        tu.local_cpt = tune->local_cpt; tu.nbr_waves = tune->nbr_waves; tu.nontemporal = tune->nontemporal;
+        tu.local_bpc = tune->local_bpc; tu.nbr_bpc = tune->nbr_bpc;
+    }
+    b.set_tuning(tu);
     e = b.admit(&j);
     if(!e.empty()) return fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
     Batch::KernelTimes kw, kt;
@@ -811,4 +853,112 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
     out->monitor_ms = kt.n_monitor ? kt.monitor_ms / kt.n_monitor : 0;
     out->bytes_per_update = (double)(lattice == 3 ? 19 : 9) * 2 * (precision == FP_F64 ? 8 : 4);
     return FP_OK;
+}
+
+extern "C" int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_per_copy)
+{
+    if(!ctx || bytes < 16 || iters < 1 || !ms_per_copy) return FP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    const int64_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    if(hipMalloc(&a, n16 * 16) != hipSuccess) { (void)hipGetLastError(); return fail(ctx, FP_ERR_OOM, "copy buffers"); }
+    if(hipMalloc(&b, n16 * 16) != hipSuccess) { (void)hipFree(a); (void)hipGetLastError(); return fail(ctx, FP_ERR_OOM, "copy buffers"); }
+    (void)hipMemsetAsync(a, 0, n16 * 16, ctx->stream);
+    kern_copy(a, b, n16, ctx->stream);                     // warm
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, ctx->stream);
+    for(int i = 0; i < iters; i++) kern_copy(a, b, n16, ctx->stream);
+    (void)hipEventRecord(e1, ctx->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if(e != hipSuccess) return fail(ctx, FP_ERR_DEVICE, hipGetErrorString(e));
+    *ms_per_copy = ms / iters;
+    return FP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: time every launch shape of the two step kernels on one resident
+// lattice and keep the fastest of each.  Run i pairs local shape i with
+// neighbour shape i mod 12 (the kernels are timed independently by events).
+// ---------------------------------------------------------------------------
+static const int TUNE_LOCAL[24][3] = {      // cells/thread, blocks/CU, non-temporal
+    {1, 2, 0}, {1, 4, 0}, {1, 8, 0}, {1, 16, 0}, {2, 2, 0}, {2, 4, 0}, {2, 8, 0}, {2, 16, 0},
+    {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {4, 16, 0}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {1, 16, 1},
+    {2, 2, 1}, {2, 4, 1}, {2, 8, 1}, {2, 16, 1}, {4, 2, 1}, {4, 4, 1}, {4, 8, 1}, {4, 16, 1}};
+static const int TUNE_NBR[12][2] = {        // segments/workgroup, blocks/CU (0: one per item)
+    {2, 0}, {2, 4}, {2, 8}, {2, 16}, {4, 0}, {4, 2}, {4, 4}, {4, 8}, {8, 0}, {8, 2}, {8, 3}, {8, 4}};
+
+extern "C" void fp_tune_shape(int kind, int i, int *a, int *b, int *c)
+{
+    if(kind == 0 && i >= 0 && i < 24) { *a = TUNE_LOCAL[i][0]; *b = TUNE_LOCAL[i][1]; *c = TUNE_LOCAL[i][2]; }
+    else if(kind == 1 && i >= 0 && i < 12) { *a = TUNE_NBR[i][0]; *b = TUNE_NBR[i][1]; *c = 0; }
+    else { *a = *b = *c = 0; }
+}
+
+extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
+                           int precision, int steps, fp_tuning *best, double *local_ms, double *nbr_ms)
+{
+    if(!ctx || !solid || !best || steps < 2 || (lattice != 2 && lattice != 3)) return FP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    Geometry g;
+    std::string e = geometry_build(g, solid, lattice, nx, ny, lattice == 3 ? nz : 1);
+    if(!e.empty()) return fail(ctx, status_of_geometry_error(e), e);
+    Job j;
+    j.geo = &g;
+    j.lattice = lattice;
+    j.prec = precision;
+    j.F[0] = 2.5e-7 * 40;
+    if(lattice == 3) {
+        fp_config3d c;
+        fp_config3d_default(&c);
+        c.eps = 0.0;
+        c.max_steps = INT64_MAX / 4;
+        j.cfg = cfg3(c, g.nn, false);
+    } else {
+        fp_config2d c;
+        fp_config2d_default(&c);
+        c.eps = 0.0; c.ksigma = 0.0; c.qfloor = 0.0;
+        c.max_steps = INT64_MAX / 4;
+        j.cfg = cfg2(c, nx, ny);
+    }
+    Batch b(lattice, precision, ctx->stream, 2, false);
+    e = b.admit(&j);
+    if(!e.empty()) { b.detach_all(); free_job_memory(j); return fail(ctx, FP_ERR_OOM, e); }
+    double lt[24], nt[12];
+    for(auto &v : lt) v = 1e30;
+    for(auto &v : nt) v = 1e30;
+    for(int i = 0; i < 24 && e.empty(); i++) {
+        Tuning t = ctx->tune;
+        t.local_cpt = TUNE_LOCAL[i][0]; t.local_bpc = TUNE_LOCAL[i][1]; t.nontemporal = TUNE_LOCAL[i][2];
+        t.nbr_waves = TUNE_NBR[i % 12][0]; t.nbr_bpc = TUNE_NBR[i % 12][1];
+        b.set_tuning(t);
+        Batch::KernelTimes kw, kt;
+        e = b.time_steps(2, kw);
+        if(e.empty()) e = b.time_steps(steps, kt);
+        if(!e.empty()) break;
+        lt[i] = std::min(lt[i], kt.n_local ? kt.local_ms / kt.n_local : 1e30);
+        nt[i % 12] = std::min(nt[i % 12], kt.n_nbr ? kt.nbr_ms / kt.n_nbr : 1e30);
+    }
+    b.detach_all();
+    free_job_memory(j);
+    if(!e.empty()) return fail(ctx, FP_ERR_DEVICE, e);
+    int bl = 0, bn = 0;
+    for(int i = 1; i < 24; i++) if(lt[i] < lt[bl]) bl = i;
+    for(int i = 1; i < 12; i++) if(nt[i] < nt[bn]) bn = i;
+    best->local_cpt = TUNE_LOCAL[bl][0];
+    best->local_bpc = TUNE_LOCAL[bl][1];
+    best->nontemporal = TUNE_LOCAL[bl][2];
+    best->nbr_waves = TUNE_NBR[bn][0];
+    best->nbr_bpc = TUNE_NBR[bn][1];
+    best->chunk_steps = ctx->chunk;
+    if(local_ms) for(int i = 0; i < 24; i++) local_ms[i] = lt[i];
+    if(nbr_ms) for(int i = 0; i < 12; i++) nbr_ms[i] = nt[i];
+    return fp_set_tuning(ctx, best);
 }

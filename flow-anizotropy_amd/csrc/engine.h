@@ -11,8 +11,9 @@ namespace fp {
 
 // kernels.hip
 void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
-void kern_step(int lattice, int prec, bool local, const InstDesc *insts, const InstState *st,
-               const void *work, int nwork, Partial *part, hipStream_t s);
+void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDesc *insts,
+               const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
+void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s);
 void kern_monitor(int ninst, const InstDesc *insts, const InstCfg *cfgs, InstState *st,
                   const Partial *lpart, const Partial *npart, hipStream_t s);
 
@@ -83,6 +84,8 @@ public:
     int prec() const { return prec_; }
     const BatchStats &stats() const { return stats_; }
     void set_chunk(int c) { chunk_ = c < 2 ? 2 : (c & ~1); dirty_ = true; }
+    void set_tuning(const Tuning &t) { tune_ = t; dirty_ = true; }
+    const Tuning &tuning() const { return tune_; }
 
     // Timing: launch `steps` steps (even) directly, with a HIP event pair on
     // the batch stream around every kernel; returns the summed kernel times
@@ -114,6 +117,7 @@ private:
     InstState *h_state_ = nullptr;       // pinned mirror
     int n_inst_ = 0, n_lwork_ = 0, n_nwork_ = 0;
     size_t cap_inst_ = 0, cap_lwork_ = 0, cap_nwork_ = 0, cap_lpart_ = 0, cap_npart_ = 0;
+    Tuning tune_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     BatchStats stats_;

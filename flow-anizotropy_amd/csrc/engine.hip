@@ -65,10 +65,12 @@ std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, i
 static int lattice_q(int lattice) { return lattice == 3 ? 19 : 9; }
 static size_t elem_bytes(int prec) { return prec == PREC_F64 ? 8 : 4; }
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+// population arrays are padded to whole 1024-cell blocks (the widest local-kernel workgroup)
+static int64_t pop_stride(int64_t n_fluid) { return round_up(n_fluid, 1024); }
 
 size_t job_bytes(const Geometry &g, int lattice, int prec, bool fields)
 {
-    const int64_t stride = round_up(g.n_fluid, 64);
+    const int64_t stride = pop_stride(g.n_fluid);
     size_t b = (size_t)lattice_q(lattice) * stride * elem_bytes(prec);
     if(fields) b += 2 * (size_t)g.dim * g.n_fluid * sizeof(float);
     return b;
@@ -149,7 +151,9 @@ std::string Batch::admit(Job *j)
 {
     if(j->lattice != lattice_ || j->prec != prec_) return "job does not match the batch type";
     const Geometry &g = *j->geo;
-    j->stride = round_up(g.n_fluid, 64);
+    j->stride = pop_stride(g.n_fluid);
+    if((uint64_t)lattice_q(lattice_) * (uint64_t)j->stride >= (1ull << 32))
+        return "lattice too large for 32-bit population offsets";
     const size_t fb = (size_t)lattice_q(lattice_) * j->stride * elem_bytes(prec_);
     hipError_t e = hipMalloc(&j->d_f, fb);
     if(e != hipSuccess) { j->d_f = nullptr; (void)hipGetLastError(); return "out of device memory"; }
@@ -214,22 +218,34 @@ std::string Batch::rebuild()
         cfg[i] = j->cfg;
         state[i] = j->state;
         const Geometry &g = *j->geo;
+        const int64_t lchunk = 256 * (int64_t)tune_.local_cpt;
         desc[i].lp0 = (int32_t)lw.size();
-        for(int64_t c = 0; c < g.n_fluid; c += 256) {
+        for(int64_t c = 0; c < g.n_fluid; c += lchunk) {
             LocalWork w;
             w.inst = i;
-            w.ncell = (int32_t)std::min<int64_t>(256, g.n_fluid - c);
+            w.ncell = (int32_t)std::min<int64_t>(lchunk, g.n_fluid - c);
             w.first = c;
             lw.push_back(w);
         }
         desc[i].lp1 = (int32_t)lw.size();
         desc[i].np0 = (int32_t)nw.size();
-        for(int64_t u = 0; u < g.nunits; u += 4) {
-            NbrWork w;
-            w.inst = i;
-            w.nunit = (int32_t)std::min<int64_t>(4, g.nunits - u);
-            w.first = u;
-            nw.push_back(w);
+        {
+            // z-slabs of nbr_waves rows; inside a slab y then segment vary fastest
+            const int64_t nwv = tune_.nbr_waves;
+            const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
+            const int64_t rows_per_plane = g.dim == 3 ? g.ny : 1;
+            const int64_t ustride = rows_per_plane * g.nseg;
+            for(int64_t z0 = 0; z0 < planes; z0 += nwv) {
+                const int32_t nu = (int32_t)std::min<int64_t>(nwv, planes - z0);
+                for(int64_t u = 0; u < ustride; u++) {
+                    NbrWork w;
+                    w.inst = i;
+                    w.nunit = nu;
+                    w.first = (int32_t)(z0 * ustride + u);
+                    w.ustride = (int32_t)ustride;
+                    nw.push_back(w);
+                }
+            }
         }
         desc[i].np1 = (int32_t)nw.size();
     }
@@ -276,7 +292,8 @@ std::string Batch::run_chunk()
     auto launch_all = [&]() {
         for(int t = 0; t < chunk_; t++) {
             const bool local = (t & 1) == 0;     // every live job starts a chunk at an even step count
-            kern_step(lattice_, prec_, local, d_inst_, d_state_, local ? (const void *)d_lwork_ : (const void *)d_nwork_,
+            kern_step(lattice_, prec_, local, tune_, d_inst_, d_state_,
+                      local ? (const void *)d_lwork_ : (const void *)d_nwork_,
                       local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
             kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lpart_, d_npart_, stream_);
         }
@@ -337,7 +354,8 @@ std::string Batch::time_steps(int steps, KernelTimes &kt)
     for(int t = 0; t < steps; t++) {
         const bool local = (t & 1) == 0;
         (void)hipEventRecord(ev[k++], stream_);
-        kern_step(lattice_, prec_, local, d_inst_, d_state_, local ? (const void *)d_lwork_ : (const void *)d_nwork_,
+        kern_step(lattice_, prec_, local, tune_, d_inst_, d_state_,
+                  local ? (const void *)d_lwork_ : (const void *)d_nwork_,
                   local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
         (void)hipEventRecord(ev[k++], stream_);
         kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lpart_, d_npart_, stream_);

@@ -20,11 +20,35 @@
 // stream_collide does.  Each location is read and written by one thread only.
 #include "collide.h"
 
+#include <algorithm>
+
 namespace fp {
 
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+
+// Pointers read out of InstDesc are generic to the compiler, which then emits
+// flat memory instructions.  These casts state the address space: global for
+// lattice data, constant for the read-only segment table (uniform reads of it
+// become scalar loads).
+template<class T> using gptr = __attribute__((address_space(1))) T *;
+template<class T> using cptr = const __attribute__((address_space(4))) T *;
+template<class T> __device__ __forceinline__ gptr<T> as_global(void *p) { return (gptr<T>)p; }
+
+// population traffic is streamed (each value is read and written once a
+// step); NT selects loads/stores that do not allocate in the caches
+template<bool NT, class V> __device__ __forceinline__ V ld(gptr<V> p)
+{
+    if constexpr(NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template<bool NT, class V> __device__ __forceinline__ void stv(gptr<V> p, V v)
+{
+    if constexpr(NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template<class T> __device__ __forceinline__ cptr<T> as_const(const void *p) { return (cptr<T>)p; }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD one
 // contiguous range of work so neighbouring segments share its L2.
@@ -42,16 +66,27 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// exact block reduction of the three chunk sums of up to three components
+// Exact flux partials.  Each thread adds its cells' chunks (fix_split) into
+// three integer-valued doubles per component; with at most 4 cells a thread
+// and 64 lanes a wavefront, every sum stays below 2^53 and is exact.  The
+// wavefront sums are widened to 64-bit integers and added across the block.
 template<int NC>
-__device__ __forceinline__ void block_flux(const double *u, int bad, Partial *out)
+__device__ __forceinline__ void fix_add(const double *u, double (&c)[3][3], int &bad)
+{
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        double a, b, d;
+        fix_split(u[k], a, b, d, bad);
+        c[k][0] += a; c[k][1] += b; c[k][2] += d;
+    }
+}
+
+template<int NC>
+__device__ __forceinline__ void block_flux(double (&c)[3][3], int bad, Partial *out)
 {
     __shared__ double sh[16][9];
     __shared__ int shbad[16];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double c[NC][3];
-#pragma unroll
-    for(int k = 0; k < NC; k++) fix_split(u[k], c[k][0], c[k][1], c[k][2], bad);
 #pragma unroll
     for(int k = 0; k < NC; k++)
 #pragma unroll
@@ -123,8 +158,8 @@ __device__ __forceinline__ void node(typename Store<L, P>::T *v, const InstDesc 
 template<class L>
 __device__ __forceinline__ void write_vel(const InstDesc &D, int slot, int64_t i, const double *u)
 {
-    float *v = D.vel[slot - 1];
-    if(!v) return;
+    if(!D.vel[slot - 1]) return;
+    gptr<float> v = as_global<float>(D.vel[slot - 1]);
 #pragma unroll
     for(int c = 0; c < L::DIM; c++) v[i * L::DIM + c] = (float)u[c];
 }
@@ -137,7 +172,7 @@ __global__ void __launch_bounds__(256)
 k_rest(InstDesc D)
 {
     using T = typename Store<L, P>::T;
-    T *f = (T *)D.f;
+    gptr<T> f = as_global<T>(D.f);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= D.stride) return;
 #pragma unroll
@@ -151,128 +186,238 @@ k_rest(InstDesc D)
     }
 }
 
+// Work items of a launch are dealt to its workgroups so that XCD x (which the
+// hardware gives every 8th workgroup) owns items [x n/8, (x+1) n/8) and its
+// workgroups walk that range together: those running at the same time on an
+// XCD work on neighbouring items and share its L2.
+struct ItemRange { int begin, end, step; };
+__device__ __forceinline__ ItemRange my_items(int nitems)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    if(G < 8) return {b, nitems, G};
+    const int x = b & 7, j = b >> 3;
+    const int gx = (G - x + 7) >> 3;
+    const int lo = (int)((int64_t)nitems * x / 8), hi = (int)((int64_t)nitems * (x + 1) / 8);
+    return {lo + j, hi, gx};
+}
+
 // ---------------------------------------------------------------------------
-// local step (odd t): one thread per compact cell
+// local step (odd t): each thread updates CPT consecutive compact cells with
+// vector loads/stores; a workgroup covers 256*CPT cells of one instance.
+// The population arrays are padded to a multiple of 1024 cells, so a vector
+// never leaves its array; padding cells keep their rest values.
 // ---------------------------------------------------------------------------
-template<class L, int P>
+template<class T, int N> struct VecT { typedef T type __attribute__((ext_vector_type(N))); };
+template<class T> struct VecT<T, 1> { typedef T type; };
+template<class T, int N> __device__ __forceinline__ T &lane_of(typename VecT<T, N>::type &v, int c)
+{
+    if constexpr(N == 1) return v;
+    else return reinterpret_cast<T *>(&v)[c];
+}
+
+template<class L, int P, int CPT, bool NT>
 __global__ void __launch_bounds__(256)
 k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
-        const LocalWork *__restrict__ work, Partial *__restrict__ part)
+        const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part)
 {
     using T = typename Store<L, P>::T;
-    const int wb = xcd_remap(blockIdx.x, gridDim.x);
-    const LocalWork w = work[wb];
-    const int phase = st[w.inst].phase;
-    if(phase >= PH_DONE) return;
-    const int flux = st[w.inst].flux_next;
-    const int vslot = st[w.inst].vel_next;
-    const InstDesc &D = insts[w.inst];
-    T *f = (T *)D.f;
-    const int64_t stride = D.stride;
+    using V = typename VecT<T, CPT>::type;
+    const ItemRange R = my_items(nwork);
+    for(int it = R.begin; it < R.end; it += R.step) {
+        const LocalWork w = work[it];
+        if(st[w.inst].phase >= PH_DONE) continue;
+        const int flux = st[w.inst].flux_next;
+        const int vslot = st[w.inst].vel_next;
+        const InstDesc &D = insts[w.inst];
+        gptr<T> f = as_global<T>(D.f);
+        const int64_t stride = D.stride;
 
-    const bool act = (int)threadIdx.x < w.ncell;
-    const int64_t i = w.first + threadIdx.x;
-    double u[3] = {0.0, 0.0, 0.0};
-    if(act) {
-        T v[L::Q];
+        const int c0 = (int)threadIdx.x * CPT;
+        const int nreal = w.ncell - c0;             // real cells of this thread (may be <= 0)
+        const int64_t i = w.first + c0;
+        double uc[CPT][3];
+        if(nreal > 0) {
+            V in[L::Q];
 #pragma unroll
-        for(int k = 0; k < L::Q; k++) v[k] = f[k * stride + i];
-        node<L, P>(v, D, u);
+            for(int k = 0; k < L::Q; k++) in[k] = ld<NT>((gptr<V>)(f + k * stride + i));
+            V out[L::Q];
 #pragma unroll
-        for(int k = 0; k < L::Q; k++) f[L::opp(k) * stride + i] = v[k];
-        if(vslot) write_vel<L>(D, vslot, i, u);
+            for(int c = 0; c < CPT; c++) {
+                T v[L::Q];
+#pragma unroll
+                for(int k = 0; k < L::Q; k++) v[k] = lane_of<T, CPT>(in[k], c);
+                node<L, P>(v, D, uc[c]);
+                const bool real = c < nreal;
+#pragma unroll
+                for(int k = 0; k < L::Q; k++)
+                    lane_of<T, CPT>(out[L::opp(k)], c) = real ? v[k] : lane_of<T, CPT>(in[L::opp(k)], c);
+            }
+#pragma unroll
+            for(int k = 0; k < L::Q; k++) stv<NT>((gptr<V>)(f + k * stride + i), out[k]);
+        }
+        if(vslot && nreal > 0) {
+#pragma unroll
+            for(int c = 0; c < CPT; c++)
+                if(c < nreal) write_vel<L>(D, vslot, i + c, uc[c]);
+        }
+        if(flux) {
+            double ch[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            int bad = 0;
+#pragma unroll
+            for(int c = 0; c < CPT; c++)
+                if(c < nreal) fix_add<L::DIM>(uc[c], ch, bad);
+            block_flux<L::DIM>(ch, bad, &part[it]);
+            __syncthreads();
+        }
     }
-    if(flux) block_flux<L::DIM>(u, 0, &part[wb]);
 }
 
 // ---------------------------------------------------------------------------
 // neighbour step (even t): one wavefront per (row, 64-cell segment), lane = x
+//
+// Every lookup of a neighbour row goes through scalar loads of its SegInfo
+// (the row and segment are the same for the whole wavefront); a lane then
+// needs one mbcnt for its neighbour's compact index.  A neighbour at x-1 or
+// x+1 is the same segment's mask shifted by one, with the single lane whose
+// neighbour sits in the next/previous segment (or across the periodic wrap)
+// patched from uniform values.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-template<class L, int P>
-__global__ void __launch_bounds__(256)
+__device__ __forceinline__ uint32_t lane_rank_plus(uint64_t m, uint32_t base)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The 9 rows (dy, dz) in {-1,0,1}^2 around a row, numbered (dy+1)*3 + (dz+1);
+// which of them a lattice reaches and whether it needs the previous / next
+// segment there (a link with dx = -1 / +1).
+template<class L> __device__ __forceinline__ constexpr int row_of(int k)
+{
+    return (-L::ey(k) + 1) * 3 + (-L::ez(k) + 1);
+}
+template<class L> __device__ __forceinline__ constexpr bool row_dx(int r, int dx)
+{
+    for(int k = 1; k < L::Q; k++)
+        if(row_of<L>(k) == r && -L::ex(k) == dx) return true;
+    return false;
+}
+template<class L> __device__ __forceinline__ constexpr bool row_used(int r)
+{
+    return row_dx<L>(r, -1) || row_dx<L>(r, 0) || row_dx<L>(r, 1);
+}
+
+template<class L, int P, bool NT>
+__global__ void __launch_bounds__(512)
 k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
-      const NbrWork *__restrict__ work, Partial *__restrict__ part)
+      const NbrWork *__restrict__ work, int nwork, Partial *__restrict__ part)
 {
     using T = typename Store<L, P>::T;
     constexpr int Q = L::Q;
-    const int wb = xcd_remap(blockIdx.x, gridDim.x);
-    const NbrWork w = work[wb];
-    const int phase = st[w.inst].phase;
-    if(phase >= PH_DONE) return;
-    const int flux = st[w.inst].flux_next;
-    const int vslot = st[w.inst].vel_next;
-    const InstDesc &D = insts[w.inst];
-    T *f = (T *)D.f;
-    const int64_t stride = D.stride;
-    const int nx = D.nx, ny = D.ny, nz = D.nz, nseg = D.nseg;
-    const SegInfo *seg = D.seg;
-
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = uni(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const bool wave_act = wave < w.nunit;
-    const int unit = (int)w.first + wave;
-    const int row = unit / nseg;
-    const int sg = unit - row * nseg;
-    const int y = (L::DIM == 3) ? row % ny : row;
-    const int z = (L::DIM == 3) ? row / ny : 0;
-    const int x = sg * 64 + lane;
+    const ItemRange R = my_items(nwork);
+    for(int it = R.begin; it < R.end; it += R.step) {
+        const NbrWork w = work[it];
+        if(st[w.inst].phase >= PH_DONE) continue;
+        const int flux = st[w.inst].flux_next;
+        const int vslot = st[w.inst].vel_next;
+        const InstDesc &D = insts[w.inst];
+        gptr<T> f = as_global<T>(D.f);
+        const uint32_t stride = (uint32_t)D.stride;    // Q * stride < 2^32 (checked on the host)
+        const int nx = D.nx, ny = D.ny, nz = D.nz, nseg = D.nseg;
+        const int lw = (nx - 1) & 63;                  // lane of x = nx - 1 in the last segment
+        cptr<SegInfo> seg = as_const<SegInfo>(D.seg);
 
-    uint64_t m0 = 0;
-    uint32_t st0 = 0;
-    if(wave_act) { m0 = seg[unit].mask; st0 = seg[unit].start; }
-    const bool act = wave_act && x < nx && ((m0 >> lane) & 1ull);
-    const uint32_t i = st0 + lane_rank(m0);
+        const bool wave_act = wave < w.nunit;
+        const int unit = uni(w.first + (wave_act ? wave : 0) * w.ustride);
+        const int row = unit / nseg;
+        const int sg = unit - row * nseg;
+        const int y = (L::DIM == 3) ? row % ny : row;
+        const int z = (L::DIM == 3) ? row / ny : 0;
+        const int sp = sg > 0 ? sg - 1 : nseg - 1;     // previous / next segment (periodic)
+        const int sn = sg < nseg - 1 ? sg + 1 : 0;
+        const int ls = sg < nseg - 1 ? 63 : lw;        // lane whose x+1 leaves the segment
 
-    double u[3] = {0.0, 0.0, 0.0};
-    if(act) {
-        // neighbour n_k = cell - e_k for k >= 1: solidity and compact index
-        bool fl[Q];
-        uint32_t nb[Q];
-        fl[0] = true; nb[0] = i;
+        const uint64_t m0 = uni64(seg[unit].mask);
+        const uint32_t i = lane_rank_plus(m0, uni(seg[unit].start));
+        const bool act = wave_act && ((m0 >> lane) & 1ull);
+
+        double u[3] = {0.0, 0.0, 0.0};
+        if(act) {
+            // segment data of the reachable neighbour rows, loaded once
+            uint64_t mc[9], mp[9], mn[9];
+            uint32_t sc[9], spv[9], snv[9];
 #pragma unroll
-        for(int k = 1; k < Q; k++) {
-            const int dx = -L::ex(k), dy = -L::ey(k), dz = -L::ez(k);
-            int yy = y + dy; yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
-            int zz = z;
-            if constexpr(L::DIM == 3) { zz = z + dz; zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz); }
-            const int rb = (zz * ny + yy) * nseg;
-            if(dx == 0) {
-                const SegInfo s = seg[rb + sg];
-                fl[k] = (s.mask >> lane) & 1ull;
-                nb[k] = s.start + lane_rank(s.mask);
-            } else {
-                int xx = x + dx; xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
-                const int b = xx & 63;
-                const SegInfo s = seg[rb + (xx >> 6)];
-                fl[k] = (s.mask >> b) & 1ull;
-                nb[k] = s.start + (uint32_t)__popcll(s.mask & ((1ull << b) - 1ull));
+            for(int r = 0; r < 9; r++) {
+                mc[r] = mp[r] = mn[r] = 0; sc[r] = spv[r] = snv[r] = 0;
+                if(!row_used<L>(r)) continue;
+                const int dy = r / 3 - 1, dz = r % 3 - 1;
+                int yy = y + dy; yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+                int zz = z;
+                if constexpr(L::DIM == 3) { zz = z + dz; zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz); }
+                const int rb = uni((zz * ny + yy) * nseg);
+                mc[r] = uni64(seg[rb + sg].mask);
+                sc[r] = uni(seg[rb + sg].start);
+                if(row_dx<L>(r, -1)) { mp[r] = uni64(seg[rb + sp].mask); spv[r] = uni(seg[rb + sp].start); }
+                if(row_dx<L>(r, 1))  { mn[r] = uni64(seg[rb + sn].mask); snv[r] = uni(seg[rb + sn].start); }
             }
-        }
-        T v[Q];
-        v[0] = f[i];
+            // off[k]: the location population k is read from this step and the
+            // location population opp(k) is written to: f[opp k][cell - e_k]
+            // when that neighbour is fluid, else (halfway bounce-back) f[k][cell]
+            uint32_t off[Q];
+            off[0] = i;
 #pragma unroll
-        for(int k = 1; k < Q; k++) {
-            const int64_t off = fl[k] ? (int64_t)L::opp(k) * stride + nb[k]
-                                      : (int64_t)k * stride + i;
-            v[k] = f[off];
-        }
-        node<L, P>(v, D, u);
-        f[i] = v[0];
+            for(int k = 1; k < Q; k++) {
+                const int r = row_of<L>(k), dx = -L::ex(k);
+                const uint32_t add = (uint32_t)L::opp(k) * stride;
+                const uint64_t m = mc[r];
+                uint64_t M;
+                uint32_t nbi;
+                if(dx == 0) {
+                    M = m;
+                    nbi = lane_rank_plus(m, sc[r] + add);
+                } else if(dx < 0) {
+                    const uint64_t prevbit = sg > 0 ? (mp[r] >> 63) : ((mp[r] >> lw) & 1ull);
+                    M = (m << 1) | prevbit;
+                    const uint32_t l0 = sg > 0 ? sc[r] - 1u
+                                               : spv[r] + (uint32_t)__popcll(mp[r] & ((1ull << lw) - 1ull));
+                    const uint32_t gen = lane_rank_plus(m << 1, sc[r] + add);
+                    nbi = lane == 0 ? l0 + add : gen;
+                } else {
+                    M = ((m >> 1) & ~(1ull << ls)) | ((mn[r] & 1ull) << ls);
+                    const uint32_t gen = lane_rank_plus(m >> 1, sc[r] + (uint32_t)(m & 1ull) + add);
+                    nbi = lane == ls ? snv[r] + add : gen;
+                }
+                off[k] = ((M >> lane) & 1ull) ? nbi : i + (uint32_t)k * stride;
+            }
+            T v[Q];
 #pragma unroll
-        for(int k = 1; k < Q; k++) {
-            const int ko = L::opp(k);
-            const int64_t off = fl[ko] ? (int64_t)k * stride + nb[ko]
-                                       : (int64_t)ko * stride + i;
-            f[off] = v[k];
+            for(int k = 0; k < Q; k++) v[k] = ld<NT>(f + off[k]);
+            node<L, P>(v, D, u);
+#pragma unroll
+            for(int k = 0; k < Q; k++) stv<NT>(f + off[L::opp(k)], v[k]);
+            if(vslot) write_vel<L>(D, vslot, (int64_t)i, u);
         }
-        if(vslot) write_vel<L>(D, vslot, (int64_t)i, u);
+        if(flux) {
+            double ch[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            int bad = 0;
+            if(act) fix_add<L::DIM>(u, ch, bad);
+            block_flux<L::DIM>(ch, bad, &part[it]);
+            __syncthreads();                           // LDS reused by the next item
+        }
     }
-    if(flux) block_flux<L::DIM>(u, 0, &part[wb]);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,17 +496,87 @@ static void launch_rest(const InstDesc &D, hipStream_t s)
     hipLaunchKernelGGL((k_rest<L, P>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D);
 }
 
-template<class L, int P>
-static void launch_step(bool local, const InstDesc *insts, const InstState *st,
-                        const void *work, int nwork, Partial *part, hipStream_t s)
+static int device_cus()
 {
+    static int cus[64] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if(dev < 0 || dev >= 64) dev = 0;
+    if(!cus[dev]) {
+        int v = 0;
+        if(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cus[dev] = v;
+    }
+    return cus[dev];
+}
+
+// workgroups launched for nwork items: blocks_per_cu resident per CU (0: one
+// workgroup per item), at least 8 (one per XCD), never more than the items
+static int grid_for(int nwork, int blocks_per_cu)
+{
+    if(blocks_per_cu <= 0) return nwork;
+    const int64_t g = (int64_t)device_cus() * blocks_per_cu;
+    return (int)std::max<int64_t>(std::min<int64_t>(nwork, g), std::min(nwork, 8));
+}
+
+template<class L, int P>
+static void launch_step(bool local, const Tuning &tu, const InstDesc *insts, const InstState *st,
+                        const void *work, int nwork0, Partial *part, hipStream_t s)
+{
+    const int nwork = nwork0;
     if(nwork <= 0) return;
-    if(local)
-        hipLaunchKernelGGL((k_local<L, P>), dim3(nwork), dim3(256), 0, s, insts, st,
-                           (const LocalWork *)work, part);
-    else
-        hipLaunchKernelGGL((k_nbr<L, P>), dim3(nwork), dim3(256), 0, s, insts, st,
-                           (const NbrWork *)work, part);
+    if(local) {
+        const LocalWork *lw = (const LocalWork *)work;
+        const dim3 g(grid_for(nwork, tu.local_bpc)), b(256);
+        if(tu.nontemporal) {
+            if(tu.local_cpt == 4)
+                hipLaunchKernelGGL((k_local<L, P, 4, true>), g, b, 0, s, insts, st, lw, nwork, part);
+            else if(tu.local_cpt == 2)
+                hipLaunchKernelGGL((k_local<L, P, 2, true>), g, b, 0, s, insts, st, lw, nwork, part);
+            else
+                hipLaunchKernelGGL((k_local<L, P, 1, true>), g, b, 0, s, insts, st, lw, nwork, part);
+        } else {
+            if(tu.local_cpt == 4)
+                hipLaunchKernelGGL((k_local<L, P, 4, false>), g, b, 0, s, insts, st, lw, nwork, part);
+            else if(tu.local_cpt == 2)
+                hipLaunchKernelGGL((k_local<L, P, 2, false>), g, b, 0, s, insts, st, lw, nwork, part);
+            else
+                hipLaunchKernelGGL((k_local<L, P, 1, false>), g, b, 0, s, insts, st, lw, nwork, part);
+        }
+    } else {
+        // the neighbour kernel's scattered partial-line traffic needs the L2:
+        // it never uses non-temporal accesses (measured 1.6-1.8x the bytes)
+        const NbrWork *nw = (const NbrWork *)work;
+        const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
+        hipLaunchKernelGGL((k_nbr<L, P, false>), g, b, 0, s, insts, st, nw, nwork, part);
+    }
+}
+
+// plain device copy, the bandwidth reference for the step kernels
+// 4 loads in flight per thread; a workgroup moves 16 KiB per iteration
+__global__ void __launch_bounds__(256)
+k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n)
+{
+    const int64_t step = (int64_t)gridDim.x * 1024;
+    for(int64_t base = (int64_t)blockIdx.x * 1024; base < n; base += step) {
+        float4 v[4];
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const int64_t i = base + j * 256 + threadIdx.x;
+            if(i < n) v[j] = a[i];
+        }
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const int64_t i = base + j * 256 + threadIdx.x;
+            if(i < n) b[i] = v[j];
+        }
+    }
+}
+
+void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s)
+{
+    const int64_t blocks = std::min<int64_t>((n16 + 1023) / 1024, 256 * 32);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)a, (float4 *)b, n16);
 }
 
 void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s)
@@ -376,16 +591,16 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s)
     }
 }
 
-void kern_step(int lattice, int prec, bool local, const InstDesc *insts, const InstState *st,
-               const void *work, int nwork, Partial *part, hipStream_t s)
+void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDesc *insts,
+               const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s)
 {
     if(lattice == 3) {
-        if(prec == PREC_F64) launch_step<D3Q19, PREC_F64>(local, insts, st, work, nwork, part, s);
-        else                 launch_step<D3Q19, PREC_F32>(local, insts, st, work, nwork, part, s);
+        if(prec == PREC_F64) launch_step<D3Q19, PREC_F64>(local, tu, insts, st, work, nwork, part, s);
+        else                 launch_step<D3Q19, PREC_F32>(local, tu, insts, st, work, nwork, part, s);
     } else {
-        if(prec == PREC_F64)      launch_step<D2Q9, PREC_F64>(local, insts, st, work, nwork, part, s);
-        else if(prec == PREC_F32) launch_step<D2Q9, PREC_F32>(local, insts, st, work, nwork, part, s);
-        else                      launch_step<D2Q9, PREC_F32_REF2D>(local, insts, st, work, nwork, part, s);
+        if(prec == PREC_F64)      launch_step<D2Q9, PREC_F64>(local, tu, insts, st, work, nwork, part, s);
+        else if(prec == PREC_F32) launch_step<D2Q9, PREC_F32>(local, tu, insts, st, work, nwork, part, s);
+        else                      launch_step<D2Q9, PREC_F32_REF2D>(local, tu, insts, st, work, nwork, part, s);
     }
 }
 

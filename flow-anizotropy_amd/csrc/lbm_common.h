@@ -402,6 +402,16 @@ struct Partial {
     int32_t pad;
 };
 
+// launch shapes of the step kernels (chosen by the autotuner)
+struct Tuning {
+    int local_cpt = 1;      // cells per thread in the local kernel: 1, 2 or 4
+    int nbr_waves = 4;      // wavefronts per workgroup in the neighbour kernel: 1..8, one
+                            // row segment each, stacked along z (along y in 2D)
+    int nontemporal = 0;    // local kernel: population loads/stores bypass cache allocation
+    int local_bpc = 8;      // resident workgroups per CU launched (each loops over items)
+    int nbr_bpc = 4;
+};
+
 // work of one workgroup of the local (even A-A) kernel: a run of compact cells
 struct LocalWork {
     int32_t inst;
@@ -409,12 +419,15 @@ struct LocalWork {
     int64_t first;
 };
 
-// work of one workgroup of the neighbour (odd A-A) kernel: consecutive
-// (row, segment) units of one instance, one wavefront each
+// work of one workgroup of the neighbour (odd A-A) kernel: nunit (row,
+// segment) units of one instance, one wavefront each, at first + w * ustride
+// (ustride = one row up in z, so the rows that share cache lines through the
+// z-links are processed together)
 struct NbrWork {
     int32_t inst;
     int32_t nunit;
-    int64_t first;
+    int32_t first;
+    int32_t ustride;
 };
 
 } // namespace fp

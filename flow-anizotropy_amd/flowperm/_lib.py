@@ -87,6 +87,11 @@ class BenchOut(C.Structure):
                 ("monitor_ms", C.c_double), ("bytes_per_update", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("local_cpt", C.c_int32), ("nbr_waves", C.c_int32), ("chunk_steps", C.c_int32),
+                ("nontemporal", C.c_int32), ("local_bpc", C.c_int32), ("nbr_bpc", C.c_int32)]
+
+
 # every symbol include/flowperm.h declares, with its ctypes signature
 _u8p = C.POINTER(C.c_uint8)
 _dp = C.POINTER(C.c_double)
@@ -122,7 +127,13 @@ SIGNATURES = {
                               C.POINTER(C.c_double)]),
     "fp_batch_set_chunk": (None, [C.c_void_p, C.c_int]),
     "fp_bench_steps": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 _dp, C.c_int, C.c_int, C.POINTER(BenchOut)]),
+                                 _dp, C.c_int, C.c_int, C.POINTER(Tuning), C.POINTER(BenchOut)]),
+    "fp_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "fp_autotune": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_int, C.POINTER(Tuning), _dp, _dp]),
+    "fp_tune_shape": (None, [C.c_int, C.c_int, _ip, _ip, _ip]),
+    "fp_get_tuning": (None, [C.c_void_p, C.POINTER(Tuning)]),
+    "fp_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _dp]),
 }
 
 _lib = None

@@ -237,8 +237,43 @@ typedef struct fp_bench_out {
     double  bytes_per_update; /* algorithmic bytes per fluid update: 2 Q s  */
 } fp_bench_out;
 
+/* Launch shapes of the step kernels; the autotuner picks them per device. */
+typedef struct fp_tuning {
+    int32_t local_cpt;        /* cells per thread, local kernel: 1, 2 or 4            */
+    int32_t nbr_waves;        /* z-stacked row segments per workgroup, neighbour
+                                 kernel: 1..8                                         */
+    int32_t chunk_steps;      /* steps per hipGraph replay (even)                     */
+    int32_t nontemporal;      /* 1: local-kernel loads/stores do not allocate in cache */
+    int32_t local_bpc;        /* workgroups launched per CU, local kernel: 0..32
+                                 (0: one workgroup per work item)                     */
+    int32_t nbr_bpc;          /* the same for the neighbour kernel                    */
+} fp_tuning;
+
+/* Set / read the launch shapes a context uses for its solves and batches. */
+int  fp_set_tuning(fp_ctx *ctx, const fp_tuning *t);
+void fp_get_tuning(fp_ctx *ctx, fp_tuning *t);
+
+/* tune: NULL = the context's shapes */
 int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
-                   int precision, const double F[3], int warmup, int steps, fp_bench_out *out);
+                   int precision, const double F[3], int warmup, int steps,
+                   const fp_tuning *tune, fp_bench_out *out);
+
+/* Autotuner: times every launch shape of the two step kernels on the given
+ * lattice (`steps` steps each), stores the fastest in *best and makes it the
+ * context's tuning.  When not NULL, local_ms / nbr_ms receive the average
+ * duration of every local / neighbour shape tried (FP_TUNE_NLOCAL /
+ * FP_TUNE_NNBR entries, in the order of fp_tune_shape). */
+#define FP_TUNE_NLOCAL 24
+#define FP_TUNE_NNBR   12
+int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
+                int precision, int steps, fp_tuning *best, double *local_ms, double *nbr_ms);
+/* The i-th shape tried by fp_autotune: kind 0 = local (cpt, bpc, nontemporal in
+ * a, b, c), kind 1 = neighbour (waves, bpc in a, b). */
+void fp_tune_shape(int kind, int i, int *a, int *b, int *c);
+
+/* Plain device-to-device copy of `bytes` (read + write), `iters` times:
+ * the measured bandwidth the step kernels are compared with. */
+int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_per_copy);
 
 #ifdef __cplusplus
 }
