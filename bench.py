@@ -52,6 +52,21 @@ def workload_mask(n, seed, kind="ellipsoids"):
     return m, por
 
 
+def measured_traffic(n_fluid):
+    """HBM bytes per step-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this lattice (copy-calibrated), or None when the
+    summary is for another lattice."""
+    p = REPO / "profiles" / "d3q19_f32_step_kernels_256cube_ellipsoids.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic-json:"):
+            d = json.loads(line.split(":", 1)[1])
+            if d["n_fluid"] == n_fluid:
+                return d["bytes_per_launch"]
+    return None
+
+
 def bench_steps(ctx, mask, prec, warmup, steps, tune=None):
     from flowperm import _lib
     nz, ny, nx = mask.shape
@@ -66,6 +81,124 @@ def bench_steps(ctx, mask, prec, warmup, steps, tune=None):
     if st != 0:
         raise RuntimeError(f"fp_bench_steps failed: {st} {ctx.last_error()}")
     return out
+
+
+# convergence flags of the batched and serving solves: the 64^3 parity
+# configuration (--ff 40 --lag 1000 --eps 1e-6, reference defaults otherwise)
+SOLVE_CFG = dict(ff=40, lag=1000, eps=1e-6, hold=2, min_steps=4000, max_steps=200000,
+                 avg_steps=2000)
+
+
+def k_matrix(row):
+    return np.array([[row[f"K_{a}{b}"] for b in "xyz"] for a in "xyz"])
+
+
+def k_error(r32, r64):
+    """max over components of |K32 - K64| / max(|K64|, |trace K64|)"""
+    K32, K64 = k_matrix(r32), k_matrix(r64)
+    return float((np.abs(K32 - K64) / np.maximum(np.abs(K64), abs(np.trace(K64)))).max())
+
+
+TUNED = {}
+
+
+def apply_tuning(ctx, name):
+    """make the autotuned launch shapes of a precision the context's"""
+    from flowperm import _lib
+    r = TUNED.get(name)
+    if r:
+        t = _lib.Tuning(r["local_cpt"], r["nbr_waves"], r["chunk_steps"], r["nontemporal"],
+                        r["local_bpc"], r["nbr_bpc"])
+        ctx.lib.fp_set_tuning(ctx.handle, C.byref(t))
+
+
+def plain_fp64(ctx, structs):
+    """the plain path: one structure at a time, its three forcing runs one
+    after another, FP64 (fp_solve3d)"""
+    from flowperm import solver
+    apply_tuning(ctx, "f64")
+    rows, t0 = [], time.perf_counter()
+    for s in structs:
+        rows.append(solver.solve_permeability3d(s, ctx=ctx, precision="f64", **SOLVE_CFG))
+    return rows, time.perf_counter() - t0
+
+
+def burst(ctx, structs, precision="f32"):
+    """every structure submitted at once to a request server: all forcing
+    runs step together in one launch per step, each retiring on its own"""
+    from flowperm.server import Server
+    apply_tuning(ctx, "f32" if precision == "f32" else "f64")
+    with Server(3, precision, ctx=ctx) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(s, **SOLVE_CFG) for s in structs]
+        res = srv.run(ids)
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+    return [res[i][0] for i in ids], dt, st
+
+
+def batched_sample(ctx, n, m, seed, workers):
+    from flowperm import workloads
+    t0 = time.time()
+    structs = workloads.generate("batched", n, seed, workers)
+    edges = [int(s.shape[0]) for s in structs]
+    log(f"batched: {n} structures, edges {edges} ({time.time() - t0:.1f} s to generate)")
+    rows32, t32, st = burst(ctx, structs)
+    log(f"batched FP32: {n} tensors in {t32:.2f} s = {n / t32:.3f} tensors/s; "
+        f"{st['fluid_updates'] / t32 / 1e6:.0f} MLUPS, {st['chunks']} chunks")
+    rows64, t64 = plain_fp64(ctx, structs[:m])
+    log(f"plain FP64: {m} tensors in {t64:.2f} s = {m / t64:.3f} tensors/s")
+    err = max(k_error(a, b) for a, b in zip(rows32[:m], rows64))
+    fluid = [int((s == 0).sum()) for s in structs]
+    steps32 = [max(r["steps_fx"], r["steps_fy"], r["steps_fz"]) for r in rows32]
+    return {
+        "structures": n, "edges": edges, "mean_fluid_cells": float(np.mean(fluid)),
+        "instances": 3 * n, "config": "ellipsoids 3:1, 15 deg jitter, porosity U(0.3, 0.8); "
+                                      "--ff 40 --lag 1000 --eps 1e-6",
+        "batched_fp32_tensors_per_s": n / t32, "batched_fp32_seconds": t32,
+        "batched_fp32_mlups": st["fluid_updates"] / t32 / 1e6,
+        "plain_fp64_structures": m, "plain_fp64_tensors_per_s": m / t64,
+        "plain_fp64_seconds": t64,
+        "speedup": (n / t32) / (m / t64),
+        "speedup_note": "batched FP32 tensors/s over the n structures / plain FP64 tensors/s "
+                        "over the first m of the same i.i.d. set",
+        "max_k_error_vs_fp64": err, "max_steps": int(max(steps32)),
+        "median_steps": float(np.median(steps32))}
+
+
+def serving_sample(ctx, n, m, seed, workers, load=0.8):
+    from flowperm import workloads
+    from flowperm.server import Server, poisson_arrivals, serve_schedule
+    t0 = time.time()
+    reqs = workloads.generate("serving", n, seed, workers)
+    edges = [int(s.shape[0]) for s in reqs]
+    log(f"serving: {n} requests, edges {edges} ({time.time() - t0:.1f} s to generate)")
+    _, tb, _ = burst(ctx, reqs)
+    capacity = n / tb
+    rate = load * capacity
+    arr = poisson_arrivals(rate, n, seed)
+    apply_tuning(ctx, "f32")
+    with Server(3, "f32", ctx=ctx) as srv:
+        out = serve_schedule(srv, reqs, arr, SOLVE_CFG)
+        st = srv.stats()
+    lat = np.array([o["latency"] for o in out])
+    span = max(o["t_done"] for o in out) - float(arr[0])
+    rows64, t64 = plain_fp64(ctx, reqs[:m])
+    err = max(k_error(o["row"], r) for o, r in zip(out[:m], rows64))
+    res = {
+        "requests": n, "edges": edges, "rate_per_s": rate, "load": load,
+        "capacity_tensors_per_s": capacity,
+        "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+        "max_latency_s": float(lat.max()), "solves_per_s": n / span,
+        "peak_device_bytes": st["bytes_peak"],
+        "plain_fp64_requests": m, "plain_fp64_solves_per_s": m / t64,
+        "max_k_error_vs_fp64": err,
+        "config": "edge log-uniform [32, 256], spheres / GRF (8, 8, 24) 50:50, porosity "
+                  "U(0.2, 0.8); Poisson arrivals at load x the burst capacity of the same set"}
+    log(f"serving: rate {rate:.3f}/s, p50 {res['p50_latency_s']:.2f} s, "
+        f"p99 {res['p99_latency_s']:.2f} s, {res['solves_per_s']:.3f} solves/s; "
+        f"plain FP64 {res['plain_fp64_solves_per_s']:.3f} solves/s")
+    return res
 
 
 def cpu_baseline(mask, budget_s=30.0):
@@ -123,6 +256,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
     ap.add_argument("--tune", default="", help="fixed launch shapes 'cpt,waves,nt,local_bpc,nbr_bpc' "
                     "for both precisions instead of running the autotuner")
+    ap.add_argument("--batched", default="16,4", help="'n,m': batched FP32 solve of n structures "
+                    "vs the plain FP64 path on the first m ('0' = skip)")
+    ap.add_argument("--serving", default="12,3", help="'n,m': n Poisson-arrival requests, plain "
+                    "FP64 on the first m ('0' = skip)")
+    ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -152,6 +290,8 @@ def main():
         log(f"autotune {name}: best {({k: rec[k] for k in autotune.FIELDS})}")
         log(f"  local ms {rec['local_ms']}")
         log(f"  neighbour ms {rec['nbr_ms']}")
+    TUNED.update(tuned)
+
     def shape(r):
         return (r["local_cpt"], r["nbr_waves"], r["nontemporal"], r["local_bpc"], r["nbr_bpc"])
     t32 = shape(tuned["f32"])
@@ -183,6 +323,12 @@ def main():
             "local_ms": o64.local_ms, "neighbour_ms": o64.nbr_ms,
             "step_kernel_GBs": o64.n_fluid * o64.bytes_per_update / (k64 / 1e3) / 1e9}
         log(f"FP64: {extra['fp64']['mlups']:.1f} MLUPS, {extra['fp64']['step_kernel_GBs']:.0f} GB/s")
+    if a.batched != "0":
+        n, m = (int(v) for v in a.batched.split(","))
+        extra["batched"] = batched_sample(ctx, n, m, a.seed, a.workers)
+    if a.serving != "0":
+        n, m = (int(v) for v in a.serving.split(","))
+        extra["serving"] = serving_sample(ctx, n, m, a.seed, a.workers)
     ctx.close()
 
     cpu = None
@@ -215,7 +361,11 @@ def main():
             "force": "2.5e-7 * 40 along x", "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(n_fluid) if a.mask == "ellipsoids" else None,
+            "traffic_note": "HBM bytes per launch (mean of the two step kernels) from rocprofv3 "
+                            "FETCH_SIZE x2 + WRITE_SIZE x1 (factors from a copy of known size); "
+                            f"algorithmic {bytes_launch} B",
             "kernel": "k_local/k_nbr<D3Q19,F32>: 152 B per fluid update x n_fluid per launch "
                       "over the mean event-timed duration of the two step kernels",
             "local_ms": round(o32.local_ms, 5), "neighbour_ms": round(o32.nbr_ms, 5),

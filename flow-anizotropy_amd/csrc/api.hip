@@ -1,6 +1,5 @@
 // api.hip -- the extern "C" surface of libflowperm.so (include/flowperm.h).
-#include "../../include/flowperm.h"
-#include "engine.h"
+#include "solve_common.h"
 
 #include <cmath>
 #include <cstring>
@@ -13,19 +12,7 @@
 
 using namespace fp;
 
-struct fp_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    Tuning tune;
-    int chunk = 200;
-};
-
-namespace {
-
-const double FX_BASE = 2.5e-07;               // LMB3d/lbm3d-perm.cpp:115, LMB2d/main.cpp:62
-const double TAU = 1.0, CS2 = 1.0 / 3.0;
-const double MU3 = CS2 * (TAU - 0.5);         // LMB3d/lbm3d-perm.cpp:114
+namespace fp {
 
 float mu2d()                                  // LMB2d/lbm.cpp:44-45
 {
@@ -242,7 +229,138 @@ const char *msg_of(int st)
 bool cfg3_ok(const fp_config3d *c) { return c && c->lag > 0; }
 bool cfg2_ok(const fp_config2d *c) { return c && c->lag > 0 && c->avg_sample > 0 && c->L0 > 0; }
 
-} // namespace
+// K and everything derived from it, LMB3d/lbm3d-perm.cpp:523-553; out->run[]
+// holds the three solves
+void assemble3d(fp_result3d *out, int nx, int ny, int nz, int64_t n_pore, double F)
+{
+    out->nx = nx; out->ny = ny; out->nz = nz;
+    out->n_pore = n_pore;
+    const double NN = (double)((int64_t)nx * ny * nz);
+    out->porosity = (double)n_pore / NN;
+    out->force = F;
+    out->mu = MU3;
+    double (*K)[3] = out->K;
+    for(int c = 0; c < 3; c++) {
+        K[c][0] = MU3 * out->run[0].q[c] / F;
+        K[c][1] = MU3 * out->run[1].q[c] / F;
+        K[c][2] = MU3 * out->run[2].q[c] / F;
+    }
+    const double trace = K[0][0] + K[1][1] + K[2][2];
+    out->k_xy = 0.5 * (K[0][1] + K[1][0]);
+    out->k_xz = 0.5 * (K[0][2] + K[2][0]);
+    out->k_yz = 0.5 * (K[1][2] + K[2][1]);
+    out->recip_xy = std::fabs(K[0][1] - K[1][0]) / std::fabs(trace);
+    out->recip_xz = std::fabs(K[0][2] - K[2][0]) / std::fabs(trace);
+    out->recip_yz = std::fabs(K[1][2] - K[2][1]) / std::fabs(trace);
+    out->recip_resid = dmax_ref(out->recip_xy, dmax_ref(out->recip_xz, out->recip_yz));
+    double Ks[3][3] = {{K[0][0], out->k_xy, out->k_xz},
+                       {out->k_xy, K[1][1], out->k_yz},
+                       {out->k_xz, out->k_yz, K[2][2]}};
+    double *val = out->k;
+    jacobi3(Ks, val, out->v);
+    out->k_mean = trace / 3.0;
+    const double dv = (val[0]-val[1])*(val[0]-val[1]) + (val[1]-val[2])*(val[1]-val[2])
+                    + (val[2]-val[0])*(val[2]-val[0]);
+    const double nrm = val[0]*val[0] + val[1]*val[1] + val[2]*val[2];
+    out->fa = nrm > 0 ? std::sqrt(0.5 * dv / nrm) : NAN;
+}
+
+// the tensor test of LMB3d/lbm3d-perm.cpp:555-572 once out->val holds the
+// solve under F (1,1,1)/sqrt(3)
+void assemble3d_val(fp_result3d *out, double F)
+{
+    const double c = 1.0 / std::sqrt(3.0);
+    const double Ks[3][3] = {{out->K[0][0], out->k_xy, out->k_xz},
+                             {out->k_xy, out->K[1][1], out->k_yz},
+                             {out->k_xz, out->k_yz, out->K[2][2]}};
+    double err = 0, qn = 0;
+    for(int r = 0; r < 3; r++) {
+        out->val_pred[r] = (Ks[r][0]*F*c + Ks[r][1]*F*c + Ks[r][2]*F*c) / MU3;
+        err += (out->val.q[r] - out->val_pred[r]) * (out->val.q[r] - out->val_pred[r]);
+        qn += out->val.q[r] * out->val.q[r];
+    }
+    out->val_err = std::sqrt(err) / (qn > 0 ? std::sqrt(qn) : 1);
+    out->validated = 1;
+}
+
+// the 2x2 tensor and derived values of LMB2d/main.cpp:314-353
+void assemble2d(fp_result2d *out, int lx, int ly, const uint8_t *solid, double F)
+{
+    int nfluid = 0;                                      // getporosity_full, lbm.cpp:188-195
+    for(int64_t i = 0; i < (int64_t)lx * ly; i++) if(solid[i] == 0) nfluid++;
+    const float por = nfluid / (float)(lx * ly);
+    out->lx = lx; out->ly = ly;
+    out->porosity = por;
+    out->force = F;
+    const float mu = mu2d();
+    out->mu = mu;
+    const fp_solve_out &rx = out->run[0], &ry = out->run[1];
+    const double k_xx = mu * rx.q[0] / F;
+    const double k_yx = mu * rx.q[1] / F;
+    const double k_xy = mu * ry.q[0] / F;
+    const double k_yy = mu * ry.q[1] / F;
+    const double k_off = 0.5 * (k_xy + k_yx);
+    const double trace = k_xx + k_yy;
+    const double recip = trace != 0.0 ? std::fabs(k_xy - k_yx) / std::fabs(trace) : NAN;
+    const double km = 0.5 * (k_xx + k_yy);
+    const double d1 = 0.5 * (k_xx - k_yy);
+    const double d2 = k_off;
+    const double r = std::sqrt(d1 * d1 + d2 * d2);
+    out->k_xx = k_xx; out->k_yx = k_yx; out->k_xy = k_xy; out->k_yy = k_yy;
+    out->k_off = k_off; out->recip_resid = recip;
+    out->k1 = km + r; out->k2 = km - r; out->k_mean = km;
+    out->anisotropy = km != 0.0 ? r / km : NAN;
+    out->theta_deg = 0.5 * std::atan2(d2, d1) * 180.0 / M_PI;
+    out->alpha_x_deg = std::atan2(rx.q[1], rx.q[0]) * 180.0 / M_PI;
+    out->alpha_y_deg = std::atan2(ry.q[1], ry.q[0]) * 180.0 / M_PI - 90.0;
+}
+
+// force of the 2D --validate DEG solve (LMB2d/main.cpp:360-364)
+void validate2d_force(double F, double deg, double Fv[2])
+{
+    const double th = deg * M_PI / 180.0;
+    Fv[0] = F * std::cos(th);
+    Fv[1] = F * std::sin(th);
+}
+
+// the 2D tensor test (LMB2d/main.cpp:365-380) once out->val holds that solve
+void assemble2d_val(fp_result2d *out, double F, double deg)
+{
+    double Fv[2];
+    validate2d_force(F, deg, Fv);
+    const double mu = out->mu;
+    const double pred_qx = (out->k_xx * Fv[0] + out->k_off * Fv[1]) / mu;
+    const double pred_qy = (out->k_off * Fv[0] + out->k_yy * Fv[1]) / mu;
+    const double qn = std::sqrt(out->val.q[0] * out->val.q[0] + out->val.q[1] * out->val.q[1]);
+    out->val_pred[0] = pred_qx; out->val_pred[1] = pred_qy;
+    out->val_err = std::sqrt((out->val.q[0] - pred_qx) * (out->val.q[0] - pred_qx) +
+                             (out->val.q[1] - pred_qy) * (out->val.q[1] - pred_qy)) / (qn > 0 ? qn : 1);
+    out->validated = 1;
+}
+
+// fieldres of a finished 2D solve from its two velocity snapshots (before
+// averaging, final), and the final field as dense [y][x] (U, V) pairs into
+// vel when not NULL
+std::string finish2d(const Job &j, int lx, int ly, int64_t avg_steps, fp_solve_out *o, float *vel)
+{
+    const Geometry &g = *j.geo;
+    const size_t nn = (size_t)lx * ly;
+    std::vector<float> snap(nn * 2), cur, c;
+    std::string ee = copy_field(j, 0, c);
+    if(!ee.empty()) return ee;
+    scatter_field(g, c, snap.data());
+    if(avg_steps > 0) {
+        cur.resize(nn * 2);
+        ee = copy_field(j, 1, c);
+        if(!ee.empty()) return ee;
+        scatter_field(g, c, cur.data());
+    } else cur = snap;
+    o->fieldres = field_residual(lx, ly, cur.data(), snap.data());
+    if(vel) std::memcpy(vel, cur.data(), nn * 2 * sizeof(float));
+    return "";
+}
+
+} // namespace fp
 
 // ===========================================================================
 extern "C" {
@@ -394,39 +512,7 @@ int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
                                                    " at step " + std::to_string(out->run[d].steps));
     int64_t np = 0;
     (void)check_mask(solid, (int64_t)nx * ny * nz, &np);
-    out->nx = nx; out->ny = ny; out->nz = nz;
-    out->n_pore = np;
-    const double NN = (double)((int64_t)nx * ny * nz);
-    out->porosity = (double)np / NN;
-    out->force = F;
-    out->mu = MU3;
-
-    // LMB3d/lbm3d-perm.cpp:523-553
-    double (*K)[3] = out->K;
-    for(int c = 0; c < 3; c++) {
-        K[c][0] = MU3 * out->run[0].q[c] / F;
-        K[c][1] = MU3 * out->run[1].q[c] / F;
-        K[c][2] = MU3 * out->run[2].q[c] / F;
-    }
-    const double trace = K[0][0] + K[1][1] + K[2][2];
-    out->k_xy = 0.5 * (K[0][1] + K[1][0]);
-    out->k_xz = 0.5 * (K[0][2] + K[2][0]);
-    out->k_yz = 0.5 * (K[1][2] + K[2][1]);
-    out->recip_xy = std::fabs(K[0][1] - K[1][0]) / std::fabs(trace);
-    out->recip_xz = std::fabs(K[0][2] - K[2][0]) / std::fabs(trace);
-    out->recip_yz = std::fabs(K[1][2] - K[2][1]) / std::fabs(trace);
-    out->recip_resid = dmax_ref(out->recip_xy, dmax_ref(out->recip_xz, out->recip_yz));
-    double Ks[3][3] = {{K[0][0], out->k_xy, out->k_xz},
-                       {out->k_xy, K[1][1], out->k_yz},
-                       {out->k_xz, out->k_yz, K[2][2]}};
-    double *val = out->k;
-    jacobi3(Ks, val, out->v);
-    out->k_mean = trace / 3.0;
-    const double dv = (val[0]-val[1])*(val[0]-val[1]) + (val[1]-val[2])*(val[1]-val[2])
-                    + (val[2]-val[0])*(val[2]-val[0]);
-    const double nrm = val[0]*val[0] + val[1]*val[1] + val[2]*val[2];
-    out->fa = nrm > 0 ? std::sqrt(0.5 * dv / nrm) : NAN;
-
+    assemble3d(out, nx, ny, nz, np, F);
     if(cfg->validate) {                                  // :555-572
         const double c = 1.0 / std::sqrt(3.0);
         const double Fv[1][3] = {{F * c, F * c, F * c}};
@@ -437,14 +523,7 @@ int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
             return fail(ctx, FP_ERR_NONFINITE, "non-finite flux in run val at step " +
                                                    std::to_string(out->val.steps));
         upd += u2;
-        double err = 0, qn = 0;
-        for(int r = 0; r < 3; r++) {
-            out->val_pred[r] = (Ks[r][0]*F*c + Ks[r][1]*F*c + Ks[r][2]*F*c) / MU3;
-            err += (out->val.q[r] - out->val_pred[r]) * (out->val.q[r] - out->val_pred[r]);
-            qn += out->val.q[r] * out->val.q[r];
-        }
-        out->val_err = std::sqrt(err) / (qn > 0 ? std::sqrt(qn) : 1);
-        out->validated = 1;
+        assemble3d_val(out, F);
     }
     out->seconds = now_seconds() - t0;
     out->fluid_updates = upd;
@@ -489,20 +568,13 @@ static int solve2d_dirs(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const
     if(!e.empty()) rc = fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
     int64_t upd = 0;
     const size_t nn = (size_t)lx * ly;
-    std::vector<float> snap(nn * 2), cur(nn * 2), c;
     for(int d = 0; d < ndir && rc == FP_OK; d++) {
         fill_out(jobs[d], &outs[d]);
         upd += jobs[d].state.steps * g.n_fluid;
         if(jobs[d].state.status != 0) continue;
-        std::string ee = copy_field(jobs[d], 0, c);
-        if(ee.empty()) scatter_field(g, c, snap.data());
-        if(ee.empty() && cfg->avg_steps > 0) {
-            ee = copy_field(jobs[d], 1, c);
-            if(ee.empty()) scatter_field(g, c, cur.data());
-        } else cur = snap;
+        std::string ee = finish2d(jobs[d], lx, ly, cfg->avg_steps, &outs[d],
+                                  vel ? vel + (size_t)d * nn * 2 : nullptr);
         if(!ee.empty()) { rc = fail(ctx, FP_ERR_DEVICE, ee); break; }
-        outs[d].fieldres = field_residual(lx, ly, cur.data(), snap.data());
-        if(vel) std::memcpy(vel + (size_t)d * nn * 2, cur.data(), nn * 2 * sizeof(float));
     }
     for(Job &j : jobs) free_job_memory(j);
     if(updates) *updates = upd;
@@ -536,57 +608,23 @@ int fp_solve2d(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const fp_confi
         if(out->run[d].status != FP_OK)
             return fail(ctx, FP_ERR_NONFINITE, std::string("non-finite flux in run ") + tags[d] +
                                                    " at step " + std::to_string(out->run[d].steps));
-    int nfluid = 0;                                      // getporosity_full, lbm.cpp:188-195
-    for(int64_t i = 0; i < (int64_t)lx * ly; i++) if(solid[i] == 0) nfluid++;
-    const float por = nfluid / (float)(lx * ly);
-    out->lx = lx; out->ly = ly;
-    out->porosity = por;
-    out->force = F;
-    const float mu = mu2d();
-    out->mu = mu;
-    const fp_solve_out &rx = out->run[0], &ry = out->run[1];
-    // main.cpp:327-353
-    const double k_xx = mu * rx.q[0] / F;
-    const double k_yx = mu * rx.q[1] / F;
-    const double k_xy = mu * ry.q[0] / F;
-    const double k_yy = mu * ry.q[1] / F;
-    const double k_off = 0.5 * (k_xy + k_yx);
-    const double trace = k_xx + k_yy;
-    const double recip = trace != 0.0 ? std::fabs(k_xy - k_yx) / std::fabs(trace) : NAN;
-    const double km = 0.5 * (k_xx + k_yy);
-    const double d1 = 0.5 * (k_xx - k_yy);
-    const double d2 = k_off;
-    const double r = std::sqrt(d1 * d1 + d2 * d2);
-    out->k_xx = k_xx; out->k_yx = k_yx; out->k_xy = k_xy; out->k_yy = k_yy;
-    out->k_off = k_off; out->recip_resid = recip;
-    out->k1 = km + r; out->k2 = km - r; out->k_mean = km;
-    out->anisotropy = km != 0.0 ? r / km : NAN;
-    out->theta_deg = 0.5 * std::atan2(d2, d1) * 180.0 / M_PI;
-    out->alpha_x_deg = std::atan2(rx.q[1], rx.q[0]) * 180.0 / M_PI;
-    out->alpha_y_deg = std::atan2(ry.q[1], ry.q[0]) * 180.0 / M_PI - 90.0;
-
+    assemble2d(out, lx, ly, solid, F);                   // main.cpp:314-353
     if(cfg->validate_deg >= 0.0) {                       // main.cpp:360-380
-        const double th = cfg->validate_deg * M_PI / 180.0;
-        const double Fx = F * std::cos(th), Fy = F * std::sin(th);
-        const double Fv[1][2] = {{Fx, Fy}};
+        double Fv2[2];
+        validate2d_force(F, cfg->validate_deg, Fv2);
+        const double Fv[1][2] = {{Fv2[0], Fv2[1]}};
         int64_t u2 = 0;
         rc = solve2d_dirs(ctx, solid, lx, ly, cfg, Fv, 1, &out->val, nullptr, &u2);
         if(rc != FP_OK) return rc;
         if(out->val.status != FP_OK)
             return fail(ctx, FP_ERR_NONFINITE, "non-finite flux while averaging run val");
         upd += u2;
-        const double pred_qx = (k_xx * Fx + k_off * Fy) / mu;
-        const double pred_qy = (k_off * Fx + k_yy * Fy) / mu;
-        const double qn = std::sqrt(out->val.q[0] * out->val.q[0] + out->val.q[1] * out->val.q[1]);
-        out->val_pred[0] = pred_qx; out->val_pred[1] = pred_qy;
-        out->val_err = std::sqrt((out->val.q[0] - pred_qx) * (out->val.q[0] - pred_qx) +
-                                 (out->val.q[1] - pred_qy) * (out->val.q[1] - pred_qy)) / (qn > 0 ? qn : 1);
-        out->validated = 1;
+        assemble2d_val(out, F, cfg->validate_deg);
     }
     out->seconds = now_seconds() - t0;
     out->fluid_updates = upd;
     ctx->err.clear();
-    return (rx.converged && ry.converged) ? FP_OK : FP_NOT_CONVERGED;
+    return (out->run[0].converged && out->run[1].converged) ? FP_OK : FP_NOT_CONVERGED;
 }
 
 } // extern "C"

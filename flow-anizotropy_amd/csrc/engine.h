@@ -52,6 +52,7 @@ struct Job {
 
 // bytes of device memory a job of this shape needs
 size_t job_bytes(const Geometry &g, int lattice, int prec, bool fields);
+size_t job_bytes_for(int64_t n_fluid, int dim, int lattice, int prec, bool fields);
 
 struct BatchStats {
     int64_t steps = 0;               // lattice steps launched (chunks * chunk length)
@@ -79,7 +80,13 @@ public:
     void release(Job *j);
     // forget every job without freeing its buffers (the caller owns them now)
     void detach_all();
-    int live() const { return (int)jobs_.size() + (int)pending_.size(); }
+    // jobs admitted and not yet retired
+    int live() const
+    {
+        int n = (int)pending_.size();
+        for(const Job *j : jobs_) n += (j && !j->done) ? 1 : 0;
+        return n;
+    }
     int lattice() const { return lattice_; }
     int prec() const { return prec_; }
     const BatchStats &stats() const { return stats_; }
@@ -98,6 +105,8 @@ public:
 
 private:
     std::string rebuild();
+    bool wants_rebuild() const;
+    int64_t work_items(const Job &j) const;
     void release_graph();
     void free_job(Job *j);
 
@@ -116,6 +125,7 @@ private:
     Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
     InstState *h_state_ = nullptr;       // pinned mirror
     int n_inst_ = 0, n_lwork_ = 0, n_nwork_ = 0;
+    int64_t work_total_ = 0, work_dead_ = 0;     // launched items, items of retired jobs
     size_t cap_inst_ = 0, cap_lwork_ = 0, cap_nwork_ = 0, cap_lpart_ = 0, cap_npart_ = 0;
     Tuning tune_;
     hipGraph_t graph_ = nullptr;

@@ -76,6 +76,13 @@ size_t job_bytes(const Geometry &g, int lattice, int prec, bool fields)
     return b;
 }
 
+size_t job_bytes_for(int64_t n_fluid, int dim, int lattice, int prec, bool fields)
+{
+    size_t b = (size_t)lattice_q(lattice) * pop_stride(n_fluid) * elem_bytes(prec);
+    if(fields) b += 2 * (size_t)dim * n_fluid * sizeof(float);
+    return b;
+}
+
 size_t Job::device_bytes() const { return geo ? job_bytes(*geo, lattice, prec, want_fields) : 0; }
 
 void f32_force_terms(const double F[3], float g45[19], float h15[19])
@@ -205,6 +212,7 @@ std::string Batch::rebuild()
     pending_.clear();
     jobs_ = live;
     n_inst_ = (int)jobs_.size();
+    work_total_ = work_dead_ = 0;
     if(n_inst_ == 0) { dirty_ = false; n_lwork_ = n_nwork_ = 0; return ""; }
 
     std::vector<InstDesc> desc(n_inst_);
@@ -252,6 +260,7 @@ std::string Batch::rebuild()
     if(lw.size() > INT32_MAX || nw.size() > INT32_MAX) return "batch too large";
     n_lwork_ = (int)lw.size();
     n_nwork_ = (int)nw.size();
+    work_total_ = (int64_t)lw.size() + (int64_t)nw.size();
     const size_t ni = (size_t)n_inst_;
     if(ni > cap_inst_ || !d_inst_) {
         (void)hipFree(d_inst_); (void)hipFree(d_cfg_); (void)hipFree(d_state_);
@@ -280,9 +289,17 @@ std::string Batch::rebuild()
     return "";
 }
 
+bool Batch::wants_rebuild() const
+{
+    // jobs that retired stay in the tables (the kernels skip them after one
+    // load of their phase) until a quarter of the launched work items are
+    // theirs, so that staggered retirements do not each cost a new graph
+    return dirty_ || !pending_.empty() || (work_total_ > 0 && 4 * work_dead_ > work_total_);
+}
+
 std::string Batch::run_chunk()
 {
-    if(dirty_) {
+    if(wants_rebuild()) {
         std::string err = rebuild();
         if(!err.empty()) return err;
     }
@@ -325,21 +342,30 @@ std::string Batch::run_chunk()
     const double t1 = now_seconds();
     for(int i = 0; i < n_inst_; i++) {
         Job *j = jobs_[i];
-        if(j->done) continue;
+        if(!j || j->done) continue;
         j->state = h_state_[i];
         if(j->state.phase >= PH_DONE) {
             j->done = true;
             j->t_done = t1;
             finished_.push_back(j);
-            dirty_ = true;
+            work_dead_ += work_items(*j);
         }
     }
     return "";
 }
 
+int64_t Batch::work_items(const Job &j) const
+{
+    const Geometry &g = *j.geo;
+    const int64_t lchunk = 256 * (int64_t)tune_.local_cpt;
+    const int64_t planes = g.dim == 3 ? g.nz : g.ny;
+    const int64_t ustride = (g.dim == 3 ? g.ny : 1) * (int64_t)g.nseg;
+    return (g.n_fluid + lchunk - 1) / lchunk + (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
+}
+
 std::string Batch::time_steps(int steps, KernelTimes &kt)
 {
-    if(dirty_) {
+    if(wants_rebuild()) {
         std::string err = rebuild();
         if(!err.empty()) return err;
     }
@@ -407,12 +433,15 @@ void Batch::detach_all()
 
 void Batch::release(Job *j)
 {
-    // the tables still name the job until the next rebuild, but its phase is
-    // final, so no kernel reads its lattice; mark the tables stale
+    // the tables may name the job until the next rebuild, but its phase is
+    // final, so no kernel reads its lattice (and no chunk is in flight here:
+    // run_chunk returns after its stream synchronisation)
+    if(!j->done) dirty_ = true;                      // a live job leaves: rebuild
     free_job(j);
     if(j->admitted) { j->geo->refs--; j->admitted = false; }
     for(Job *&p : jobs_) if(p == j) p = nullptr;     // the caller may now delete it
-    dirty_ = true;
+    pending_.erase(std::remove(pending_.begin(), pending_.end(), j), pending_.end());
+    finished_.erase(std::remove(finished_.begin(), finished_.end(), j), finished_.end());
 }
 
 } // namespace fp

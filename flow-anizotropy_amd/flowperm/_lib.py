@@ -14,6 +14,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libflowperm.so"
 
 FP_OK = 0
+FP_PENDING = 1
 FP_NOT_CONVERGED = 3
 FP_ERR_BAD_DIMS = 10
 FP_ERR_NON_BINARY = 11
@@ -92,6 +93,23 @@ class Tuning(C.Structure):
                 ("nontemporal", C.c_int32), ("local_bpc", C.c_int32), ("nbr_bpc", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("lattice", C.c_int32), ("precision", C.c_int32), ("budget_bytes", C.c_int64),
+                ("chunk_steps", C.c_int32), ("pad", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("device_bytes", C.c_int64), ("fluid_updates", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in
+                ("submitted", "admitted", "completed", "rejected", "queued", "resident",
+                 "live_instances", "chunks", "steps", "fluid_updates", "bytes_in_use",
+                 "bytes_peak", "budget_bytes")] + [("busy_seconds", C.c_double)]
+
+
 # every symbol include/flowperm.h declares, with its ctypes signature
 _u8p = C.POINTER(C.c_uint8)
 _dp = C.POINTER(C.c_double)
@@ -134,6 +152,18 @@ SIGNATURES = {
     "fp_tune_shape": (None, [C.c_int, C.c_int, _ip, _ip, _ip]),
     "fp_get_tuning": (None, [C.c_void_p, C.POINTER(Tuning)]),
     "fp_bench_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _dp]),
+    "fp_now": (C.c_double, []),
+    "fp_server_create": (C.c_void_p, [C.c_void_p, C.POINTER(ServerConfig), _ip]),
+    "fp_server_destroy": (None, [C.c_void_p]),
+    "fp_server_submit3d": (C.c_int64, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(Config3D)]),
+    "fp_server_submit2d": (C.c_int64, [C.c_void_p, _u8p, C.c_int, C.c_int, C.POINTER(Config2D)]),
+    "fp_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "fp_server_poll3d": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result3D),
+                                   C.POINTER(RequestInfo)]),
+    "fp_server_poll2d": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result2D),
+                                   C.POINTER(RequestInfo)]),
+    "fp_server_get_stats": (None, [C.c_void_p, C.POINTER(ServerStats)]),
 }
 
 _lib = None

@@ -236,7 +236,7 @@ def grf(shape=(128, 128, 128), porosity=0.2, corr=(8.0, 8.0, 24.0), seed=0,
         g = g.reshape(sh)
         filt = g if filt is None else filt * g
     spec *= filt
-    field = np.fft.irfftn(spec, s=shape).astype(np.float32)
+    field = np.fft.irfftn(spec, s=shape, axes=list(range(dim))).astype(np.float32)
     del spec
     t = np.quantile(field, porosity)
     solid = (field > t).astype(np.uint8)              # pore fraction = porosity

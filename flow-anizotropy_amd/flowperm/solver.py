@@ -199,8 +199,18 @@ def solve_permeability3d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
                                 C.byref(r), _lib.f32ptr(vel))
     if st not in (_lib.FP_OK, _lib.FP_NOT_CONVERGED):
         _raise(ctx, st)
-    row = {"filename": filename if filename is not None else name,
-           "ff": c.ff, "force": r.force, "mu": r.mu, "nx": nx, "ny": ny, "nz": nz,
+    row = row3d(r, st, c.ff, filename if filename is not None else name)
+    if velocity:
+        row["velocity"] = vel
+    return row
+
+
+def row3d(r, st, ff, filename):
+    """The lbm3d-perm CSV row (LMB3d/lbm3d-perm.cpp:574-615) of a Result3D, as
+    a dict with the reference's column names, plus exit_code, status, timing
+    and the --validate values."""
+    row = {"filename": filename,
+           "ff": ff, "force": r.force, "mu": r.mu, "nx": r.nx, "ny": r.ny, "nz": r.nz,
            "porosity": r.porosity}
     for d, t in enumerate(_TAGS):
         row[f"steps_{t}"] = int(r.run[d].steps)
@@ -232,8 +242,6 @@ def solve_permeability3d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
         row["val_err"] = r.val_err
         row["val_steps"] = int(r.val.steps)
         row["val_converged"] = int(r.val.converged)
-    if velocity:
-        row["velocity"] = vel
     return row
 
 
@@ -284,9 +292,18 @@ def solve_permeability2d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
                                 _lib.f32ptr(vel))
     if st not in (_lib.FP_OK, _lib.FP_NOT_CONVERGED):
         _raise(ctx, st)
+    row = row2d(r, st, c.ff, filename if filename is not None else name)
+    if velocity:
+        row["velocity"] = vel
+    return row
+
+
+def row2d(r, st, ff, filename):
+    """The lbm2d-perm CSV row (LMB2d/main.cpp:382-411) of a Result2D as a dict,
+    plus exit_code, status, timing and the --validate values."""
     rx, ry = r.run[0], r.run[1]
-    row = {"filename": filename if filename is not None else name,
-           "ff": c.ff, "force": r.force, "mu": r.mu, "porosity": r.porosity,
+    row = {"filename": filename,
+           "ff": ff, "force": r.force, "mu": r.mu, "porosity": r.porosity,
            "steps_fx": int(rx.steps), "conv_fx": int(rx.converged),
            "steps_fy": int(ry.steps), "conv_fy": int(ry.converged),
            "qx_fx": rx.q[0], "qy_fx": rx.q[1], "qx_fy": ry.q[0], "qy_fy": ry.q[1],
@@ -304,6 +321,4 @@ def solve_permeability2d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
         row["val_err"] = r.val_err
         row["val_steps"] = int(r.val.steps)
         row["val_converged"] = int(r.val.converged)
-    if velocity:
-        row["velocity"] = vel
     return row

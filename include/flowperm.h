@@ -30,10 +30,11 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 1
+#define FP_ABI_VERSION 2
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
+    FP_PENDING         = 1,   /* server request not finished yet (no exit code) */
     FP_NOT_CONVERGED   = 3,   /* a direction hit max_steps; row kept (exit 3) */
     FP_ERR_BAD_DIMS    = 10,  /* read_structure: bad dimensions      (exit 2) */
     FP_ERR_NON_BINARY  = 11,  /* read_structure: non-binary voxel    (exit 2) */
@@ -270,6 +271,78 @@ int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int l
 /* The i-th shape tried by fp_autotune: kind 0 = local (cpt, bpc, nontemporal in
  * a, b, c), kind 1 = neighbour (waves, bpc in a, b). */
 void fp_tune_shape(int kind, int i, int *a, int *b, int *c);
+
+/* ------------------------------------------------------------------------ */
+/* request server (no reference counterpart: the reference's drivers run one
+ * solver process per structure from a process pool,
+ * scripts/solve_permeability3d.py:250, and a structure's three forcing runs
+ * one after another inside it, LMB3d/lbm3d-perm.cpp:518-521).
+ *
+ * A request is one whole lbm3d-perm / lbm2d-perm invocation: its 3 (2D: 2)
+ * forcing runs, plus the --validate run when asked, become instances of one
+ * continuously batched stepper.  Requests wait in a FIFO until the device
+ * memory they need (segment table + population arrays) fits under the budget;
+ * admitted requests join the running batch at the next chunk boundary, each
+ * instance retires on the device when its own convergence test passes, and a
+ * request's memory is returned as soon as its last instance retires.
+ * A request's result does not depend on what else is resident (every instance
+ * is stepped by the same kernels on its own arrays, and flux sums are exact).
+ *
+ * Threads: fp_server_submit* and fp_server_poll* may be called from any
+ * thread while one thread drives fp_server_step.                           */
+/* ------------------------------------------------------------------------ */
+typedef struct fp_server fp_server;
+
+typedef struct fp_server_config {
+    int32_t lattice;          /* 3 (D3Q19) or 2 (D2Q9)                          */
+    int32_t precision;        /* enum fp_precision, for every request            */
+    int64_t budget_bytes;     /* device memory for resident requests; 0 = 90% of
+                                 the free memory when the server is created      */
+    int32_t chunk_steps;      /* steps between admissions (even); 0 = context's  */
+    int32_t pad;
+} fp_server_config;
+
+typedef struct fp_request_info {
+    double  t_submit;         /* fp_now() when submitted                         */
+    double  t_admit;          /* when its memory was granted (0 while queued)    */
+    double  t_done;           /* when its last instance retired (0 while pending) */
+    int64_t device_bytes;     /* footprint charged against the budget            */
+    int64_t fluid_updates;    /* fluid-cell updates of its instances             */
+} fp_request_info;
+
+typedef struct fp_server_stats {
+    int64_t submitted, admitted, completed, rejected;
+    int64_t queued;           /* waiting for memory                              */
+    int64_t resident;         /* admitted, not finished                          */
+    int64_t live_instances;   /* forcing runs still stepping                     */
+    int64_t chunks, steps, fluid_updates;
+    int64_t bytes_in_use, bytes_peak, budget_bytes;
+    double  busy_seconds;     /* wall time spent stepping on the device          */
+} fp_server_stats;
+
+/* seconds on the library's monotonic clock (the fp_request_info times) */
+double     fp_now(void);
+fp_server *fp_server_create(fp_ctx *ctx, const fp_server_config *cfg, int *status);
+void       fp_server_destroy(fp_server *s);
+/* Queue one lbm3d-perm invocation (cfg->precision must be the server's;
+ * cfg->concurrent is implied).  Returns a request id >= 0, or -status: mask
+ * errors as fp_check_mask3d, FP_ERR_OOM when the request alone exceeds the
+ * budget. */
+int64_t    fp_server_submit3d(fp_server *s, const uint8_t *solid, int nx, int ny, int nz,
+                              const fp_config3d *cfg);
+int64_t    fp_server_submit2d(fp_server *s, const uint8_t *solid, int lx, int ly,
+                              const fp_config2d *cfg);
+/* Admit what fits, step every resident instance for one chunk, retire and
+ * assemble finished requests.  *pending (may be NULL) = requests queued or
+ * resident afterwards; 0 means the server is idle.  Returns FP_OK or a device
+ * error (every resident request then fails with it). */
+int        fp_server_step(fp_server *s, int64_t *pending);
+/* FP_PENDING while queued or stepping; otherwise the request's final status
+ * (as fp_solve3d / fp_solve2d would return) with *out filled, after which the
+ * id is forgotten (FP_ERR_NOT_FOUND).  info may be NULL. */
+int        fp_server_poll3d(fp_server *s, int64_t id, fp_result3d *out, fp_request_info *info);
+int        fp_server_poll2d(fp_server *s, int64_t id, fp_result2d *out, fp_request_info *info);
+void       fp_server_get_stats(fp_server *s, fp_server_stats *out);
 
 /* Plain device-to-device copy of `bytes` (read + write), `iters` times:
  * the measured bandwidth the step kernels are compared with. */

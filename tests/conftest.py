@@ -56,7 +56,9 @@ def printed_tol(ref):
 
 
 def assert_printed_close(ours, ref, floor=0.0, what=""):
-    tol = max(printed_tol(ref), floor)
+    """|ours - true reference value| <= floor, where the reference value is
+    known only to half a unit of its 10th printed digit: the two bounds add."""
+    tol = printed_tol(ref) + floor
     assert abs(ours - ref) <= tol, f"{what}: ours {ours!r} ref {ref!r} |d|={abs(ours-ref):.3e} tol {tol:.3e}"
 
 

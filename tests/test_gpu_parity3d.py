@@ -55,10 +55,20 @@ def test_fp64_matches_reference_rows(gpu_ctx, name):
     assert r["exit_code"] == meta["returncode"]
     K = np.array([[r[f"K_{a}{b}"] for b in AX] for a in AX])
     trace = abs(np.trace(K))
+    qref = {t: float(np.linalg.norm([row[f"q{a}_{t}"] for a in AX])) for t in TAGS}
+    qtop = max(qref.values())
     for t in TAGS:
+        qn = float(np.linalg.norm([r[f"q{a}_{t}"] for a in AX]))
+        if qref[t] < 1e-9 * qtop:
+            # blocked direction (a slit forced across its walls): the flux is
+            # rounding noise in both solvers, ~1e-17 against ~1e-6, and when
+            # the drift test passes on noise depends on how the noise was
+            # summed (the reference adds sequentially under -ffast-math, this
+            # solver exactly), so only the noise level is compared
+            assert qn < 1e-9 * qtop, t
+            continue
         assert r[f"steps_{t}"] == row[f"steps_{t}"], t
         assert r[f"conv_{t}"] == row[f"conv_{t}"], t
-        qn = float(np.linalg.norm([r[f"q{a}_{t}"] for a in AX]))
         for a in AX:
             assert_printed_close(r[f"q{a}_{t}"], row[f"q{a}_{t}"], 1e-10 * qn, f"q{a}_{t}")
             ref_se = row[f"se_q{a}_{t}"]

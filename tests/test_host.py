@@ -1,0 +1,135 @@
+"""
+Host-side logic that needs no GPU: the CSV text the reference prints, the
+structure file formats and their checks, the mask generators' percolation
+test, and the serving arrival process.
+"""
+import csv
+from collections import deque
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, GOLDEN_ALL
+
+
+@pytest.mark.parametrize("name", GOLDEN_ALL[:5])
+def test_csv_text_reproduces_reference_rows(name):
+    """header and row text exactly as lbm3d-perm wrote them
+    (LMB3d/lbm3d-perm.cpp:575-615, setprecision(10))"""
+    from flowperm.solver import COLUMNS3D, _INT_COLS, csv_header, csv_row
+    lines = (GOLDEN / f"{name}.csv").read_text().splitlines(keepends=True)
+    assert lines[0] == csv_header(COLUMNS3D)
+    fields = next(csv.reader([lines[1]]))
+    row = {}
+    for c, v in zip(COLUMNS3D, fields):
+        row[c] = v if c == "filename" else (int(v) if c in _INT_COLS else float(v))
+    assert csv_row(row, COLUMNS3D) == lines[1]
+
+
+def test_raw_round_trip_and_reader_checks(tmp_path):
+    """the reader's rejections (LMB3d/lbm3d-perm.cpp:174-193; the
+    scripts/structure_io3d.py:75-127 self-test)"""
+    from flowperm import structio
+    rng = np.random.default_rng(0)
+    m = (rng.random((5, 6, 7)) > 0.5).astype(np.uint8)
+    p = structio.save_raw(tmp_path / "a.raw", m)
+    raw = p.read_bytes()
+    assert np.frombuffer(raw[:12], "<i4").tolist() == [7, 6, 5]
+    assert np.array_equal(structio.load_raw(p), m)
+    bad = {"trunc.raw": raw[:-1], "trail.raw": raw + b"\0", "hdr.raw": raw[:8],
+           "dims.raw": np.array([0, 6, 5], "<i4").tobytes() + raw[12:]}
+    for fn, data in bad.items():
+        (tmp_path / fn).write_bytes(data)
+        with pytest.raises(structio.StructureError):
+            structio.load_raw(tmp_path / fn)
+    nb = bytearray(raw)
+    nb[20] = 2
+    (tmp_path / "nb.raw").write_bytes(bytes(nb))
+    with pytest.raises(structio.StructureError, match="non-binary"):
+        structio.load_raw(tmp_path / "nb.raw")
+    structio.save_raw(tmp_path / "solid.raw", np.ones((2, 2, 2), np.uint8))
+    with pytest.raises(structio.StructureError, match="no pore"):
+        structio.load_raw(tmp_path / "solid.raw")
+
+
+def test_gif_round_trip(tmp_path):
+    """pore = red > 0 (LMB2d/lbm.cpp:412-426); array [y, x]"""
+    pytest.importorskip("PIL")
+    from flowperm import masks, structio
+    m = masks.ellipses((40, 56), 0.6, seed=3)
+    structio.save_gif2d(tmp_path / "m.gif", m)
+    assert np.array_equal(structio.load_structure2d(tmp_path / "m.gif"), m)
+
+
+def _brute_wraps(sol):
+    """BFS over the lifted torus: an axis wraps when a cluster reaches a copy
+    of one of its own cells displaced along it"""
+    pore = sol == 0
+    shp, dim = sol.shape, sol.ndim
+    lift, wr = {}, np.zeros(dim, bool)
+    for start in zip(*np.nonzero(pore)):
+        if start in lift:
+            continue
+        lift[start] = np.zeros(dim, int)
+        dq = deque([start])
+        while dq:
+            c = dq.popleft()
+            for ax in range(dim):
+                for s in (-1, 1):
+                    nc, L = list(c), lift[c].copy()
+                    nc[ax] += s
+                    if nc[ax] < 0:
+                        nc[ax] += shp[ax]; L[ax] -= 1
+                    elif nc[ax] >= shp[ax]:
+                        nc[ax] -= shp[ax]; L[ax] += 1
+                    nc = tuple(nc)
+                    if not pore[nc]:
+                        continue
+                    if nc in lift:
+                        wr |= lift[nc] != L
+                    else:
+                        lift[nc] = L
+                        dq.append(nc)
+    return tuple(bool(v) for v in wr[::-1])
+
+
+def test_percolation_matches_brute_force_on_the_torus():
+    from flowperm import masks
+    rng = np.random.default_rng(1)
+    for _ in range(120):
+        shp = tuple(int(v) for v in rng.integers(3, 8, size=3))
+        sol = (rng.random(shp) > rng.uniform(0.3, 0.7)).astype(np.uint8)
+        assert masks.percolates(sol) == _brute_wraps(sol)
+    # a straight channel along x percolates along x only
+    s = np.ones((6, 6, 6), np.uint8)
+    s[2, 3, :] = 0
+    assert masks.percolates(s) == (True, False, False)
+
+
+def test_generators_hit_their_porosity_and_percolate():
+    from flowperm import masks
+    m = masks.spheres((32, 32, 32), 0.6, 3, 6, seed=0)
+    assert abs((1 - m.mean()) - 0.6) < 0.03
+    e = masks.ellipsoids((32, 32, 32), 0.5, seed=1)
+    assert abs((1 - e.mean()) - 0.5) < 0.05
+    g = masks.grf((48, 48, 48), porosity=0.3, corr=(4, 4, 8), seed=2)
+    assert 0 < 1 - g.mean() <= 0.3 + 1e-3
+    assert any(masks.percolates(g))
+    # percolating_cluster keeps a single wrapping cluster
+    from scipy import ndimage
+    lab, n = ndimage.label(g == 0)
+    assert n >= 1
+
+
+def test_slit_mask_layout():
+    from flowperm import masks
+    s = masks.slit(8, 3)
+    assert s.shape == (8, 8, 8)
+    assert int((s == 0).all(axis=(1, 2)).sum()) == 3      # h fluid z-planes
+
+
+def test_poisson_arrivals_rate():
+    from flowperm.server import poisson_arrivals
+    a = poisson_arrivals(20.0, 4000, seed=0)
+    assert np.all(np.diff(a) > 0)
+    assert abs(len(a) / a[-1] - 20.0) < 1.0
