@@ -67,7 +67,7 @@ def measured_traffic(n_fluid):
     return None
 
 
-def bench_steps(ctx, mask, prec, warmup, steps, tune=None):
+def bench_steps(ctx, mask, prec, warmup, steps, tune=None, flux=0):
     from flowperm import _lib
     nz, ny, nx = mask.shape
     out = _lib.BenchOut()
@@ -77,7 +77,7 @@ def bench_steps(ctx, mask, prec, warmup, steps, tune=None):
         t = _lib.Tuning(tune[0], tune[1], 200, tune[2], tune[3], tune[4])
     st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, 3, prec,
                                 _lib.f64ptr(F), warmup, steps, C.byref(t) if t else None,
-                                C.byref(out))
+                                int(flux), C.byref(out))
     if st != 0:
         raise RuntimeError(f"fp_bench_steps failed: {st} {ctx.last_error()}")
     return out
@@ -137,7 +137,9 @@ def burst(ctx, structs, precision="f32"):
     return [res[i][0] for i in ids], dt, st
 
 
-def batched_sample(ctx, n, m, seed, workers):
+def batched_sample(ctx, n, seed, workers):
+    """n structures of the batched set: all 3n forcing runs stepped together
+    in FP32 by one request server, against the plain FP64 path on the same n"""
     from flowperm import workloads
     t0 = time.time()
     structs = workloads.generate("batched", n, seed, workers)
@@ -146,27 +148,32 @@ def batched_sample(ctx, n, m, seed, workers):
     rows32, t32, st = burst(ctx, structs)
     log(f"batched FP32: {n} tensors in {t32:.2f} s = {n / t32:.3f} tensors/s; "
         f"{st['fluid_updates'] / t32 / 1e6:.0f} MLUPS, {st['chunks']} chunks")
-    rows64, t64 = plain_fp64(ctx, structs[:m])
-    log(f"plain FP64: {m} tensors in {t64:.2f} s = {m / t64:.3f} tensors/s")
-    err = max(k_error(a, b) for a, b in zip(rows32[:m], rows64))
+    rows64, t64 = plain_fp64(ctx, structs)
+    upd64 = sum(r["fluid_updates"] for r in rows64)
+    log(f"plain FP64: {n} tensors in {t64:.2f} s = {n / t64:.3f} tensors/s; "
+        f"{upd64 / t64 / 1e6:.0f} MLUPS")
+    err = max(k_error(a, b) for a, b in zip(rows32, rows64))
     fluid = [int((s == 0).sum()) for s in structs]
     steps32 = [max(r["steps_fx"], r["steps_fy"], r["steps_fz"]) for r in rows32]
+    same_steps = sum(int(all(a[f"steps_{t}"] == b[f"steps_{t}"] for t in ("fx", "fy", "fz")))
+                     for a, b in zip(rows32, rows64))
     return {
         "structures": n, "edges": edges, "mean_fluid_cells": float(np.mean(fluid)),
         "instances": 3 * n, "config": "ellipsoids 3:1, 15 deg jitter, porosity U(0.3, 0.8); "
                                       "--ff 40 --lag 1000 --eps 1e-6",
         "batched_fp32_tensors_per_s": n / t32, "batched_fp32_seconds": t32,
         "batched_fp32_mlups": st["fluid_updates"] / t32 / 1e6,
-        "plain_fp64_structures": m, "plain_fp64_tensors_per_s": m / t64,
-        "plain_fp64_seconds": t64,
-        "speedup": (n / t32) / (m / t64),
-        "speedup_note": "batched FP32 tensors/s over the n structures / plain FP64 tensors/s "
-                        "over the first m of the same i.i.d. set",
-        "max_k_error_vs_fp64": err, "max_steps": int(max(steps32)),
-        "median_steps": float(np.median(steps32))}
+        "plain_fp64_tensors_per_s": n / t64, "plain_fp64_seconds": t64,
+        "plain_fp64_mlups": upd64 / t64 / 1e6,
+        "speedup": t64 / t32,
+        "max_k_error_vs_fp64": err, "structures_with_fp64_step_counts": same_steps,
+        "max_steps": int(max(steps32)), "median_steps": float(np.median(steps32))}
 
 
-def serving_sample(ctx, n, m, seed, workers, load=0.8):
+def serving_sample(ctx, n, seed, workers, load=0.8):
+    """n requests of the serving set: the burst capacity (all submitted at
+    once), then Poisson arrivals at load x that capacity with latencies, and
+    the plain FP64 sequential path on the same n"""
     from flowperm import workloads
     from flowperm.server import Server, poisson_arrivals, serve_schedule
     t0 = time.time()
@@ -183,21 +190,20 @@ def serving_sample(ctx, n, m, seed, workers, load=0.8):
         st = srv.stats()
     lat = np.array([o["latency"] for o in out])
     span = max(o["t_done"] for o in out) - float(arr[0])
-    rows64, t64 = plain_fp64(ctx, reqs[:m])
-    err = max(k_error(o["row"], r) for o, r in zip(out[:m], rows64))
+    rows64, t64 = plain_fp64(ctx, reqs)
+    err = max(k_error(o["row"], r) for o, r in zip(out, rows64))
     res = {
         "requests": n, "edges": edges, "rate_per_s": rate, "load": load,
-        "capacity_tensors_per_s": capacity,
+        "capacity_solves_per_s": capacity,
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-        "max_latency_s": float(lat.max()), "solves_per_s": n / span,
+        "max_latency_s": float(lat.max()), "served_solves_per_s": n / span,
         "peak_device_bytes": st["bytes_peak"],
-        "plain_fp64_requests": m, "plain_fp64_solves_per_s": m / t64,
+        "plain_fp64_solves_per_s": n / t64, "capacity_vs_plain_fp64": capacity * t64 / n,
         "max_k_error_vs_fp64": err,
         "config": "edge log-uniform [32, 256], spheres / GRF (8, 8, 24) 50:50, porosity "
                   "U(0.2, 0.8); Poisson arrivals at load x the burst capacity of the same set"}
-    log(f"serving: rate {rate:.3f}/s, p50 {res['p50_latency_s']:.2f} s, "
-        f"p99 {res['p99_latency_s']:.2f} s, {res['solves_per_s']:.3f} solves/s; "
-        f"plain FP64 {res['plain_fp64_solves_per_s']:.3f} solves/s")
+    log(f"serving: capacity {capacity:.3f}/s, rate {rate:.3f}/s, p50 {res['p50_latency_s']:.2f} s, "
+        f"p99 {res['p99_latency_s']:.2f} s; plain FP64 {res['plain_fp64_solves_per_s']:.3f} solves/s")
     return res
 
 
@@ -256,10 +262,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
     ap.add_argument("--tune", default="", help="fixed launch shapes 'cpt,waves,nt,local_bpc,nbr_bpc' "
                     "for both precisions instead of running the autotuner")
-    ap.add_argument("--batched", default="16,4", help="'n,m': batched FP32 solve of n structures "
-                    "vs the plain FP64 path on the first m ('0' = skip)")
-    ap.add_argument("--serving", default="12,3", help="'n,m': n Poisson-arrival requests, plain "
-                    "FP64 on the first m ('0' = skip)")
+    ap.add_argument("--batched", type=int, default=12, help="structures in the batched-solve "
+                    "sample, solved batched FP32 and plain FP64 (0 = skip)")
+    ap.add_argument("--serving", type=int, default=10, help="requests in the serving sample (0 = skip)")
     ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
     a = ap.parse_args()
 
@@ -307,6 +312,14 @@ def main():
         f"step kernels {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.3f} of peak")
 
     extra = {}
+    of = bench_steps(ctx, mask, 1, warm, max(2, (steps // 4) & ~1), t32, flux=1)
+    extra["fp32_flux_every_step"] = {
+        "mlups": of.n_fluid * of.steps / (of.total_ms / 1e3) / 1e6,
+        "ms_per_step": of.total_ms / of.steps, "local_ms": of.local_ms, "neighbour_ms": of.nbr_ms,
+        "monitor_ms": of.monitor_ms,
+        "note": "the averaging phase: exact flux sum and convergence update on every step"}
+    log(f"FP32 with the flux on every step: {extra['fp32_flux_every_step']['mlups']:.1f} MLUPS; "
+        f"local {of.local_ms:.4f} ms, neighbour {of.nbr_ms:.4f} ms, monitor {of.monitor_ms:.4f} ms")
     copy_ms = C.c_double(0)
     copy_bytes = 2 * 1024 ** 3
     if ctx.lib.fp_bench_copy(ctx.handle, copy_bytes, 20, C.byref(copy_ms)) == 0:
@@ -323,12 +336,10 @@ def main():
             "local_ms": o64.local_ms, "neighbour_ms": o64.nbr_ms,
             "step_kernel_GBs": o64.n_fluid * o64.bytes_per_update / (k64 / 1e3) / 1e9}
         log(f"FP64: {extra['fp64']['mlups']:.1f} MLUPS, {extra['fp64']['step_kernel_GBs']:.0f} GB/s")
-    if a.batched != "0":
-        n, m = (int(v) for v in a.batched.split(","))
-        extra["batched"] = batched_sample(ctx, n, m, a.seed, a.workers)
-    if a.serving != "0":
-        n, m = (int(v) for v in a.serving.split(","))
-        extra["serving"] = serving_sample(ctx, n, m, a.seed, a.workers)
+    if a.batched > 0:
+        extra["batched"] = batched_sample(ctx, a.batched, a.seed, a.workers)
+    if a.serving > 0:
+        extra["serving"] = serving_sample(ctx, a.serving, a.seed, a.workers)
     ctx.close()
 
     cpu = None

@@ -425,6 +425,13 @@ void fp_jacobi3(const double A9[9], double val[3], double vec9[9])
     std::memcpy(vec9, vec, sizeof(vec));
 }
 
+int fp_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64_t *mismatches)
+{
+    if(!solid || !mismatches || (dim != 2 && dim != 3)) return FP_ERR_ARG;
+    std::string e = geometry_verify_links(solid, dim, nx, ny, dim == 3 ? nz : 1, mismatches);
+    return e.empty() ? FP_OK : status_of_geometry_error(e);
+}
+
 int fp_check_mask3d(const uint8_t *solid, int nx, int ny, int nz, int64_t *n_pore)
 {
     if(nx <= 0 || ny <= 0 || nz <= 0) return FP_ERR_BAD_DIMS;
@@ -836,7 +843,7 @@ extern "C" void fp_get_tuning(fp_ctx *ctx, fp_tuning *t)
 
 extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
                               int precision, const double F[3], int warmup, int steps,
-                              const fp_tuning *tune, fp_bench_out *out)
+                              const fp_tuning *tune, int flux_every_step, fp_bench_out *out)
 {
     if(tune && !tuning_ok(tune)) return FP_ERR_ARG;
     if(!ctx || !solid || !out || (lattice != 2 && lattice != 3) || steps < 2 || warmup < 0)
@@ -856,12 +863,14 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
         fp_config3d_default(&c);
         c.eps = 0.0;                                  // never converges: pure stepping
         c.max_steps = INT64_MAX / 4;
+        if(flux_every_step) c.lag = 1;                // a convergence check every step
         j.cfg = cfg3(c, g.nn, false);
     } else {
         fp_config2d c;
         fp_config2d_default(&c);
         c.eps = 0.0; c.ksigma = 0.0; c.qfloor = 0.0;
         c.max_steps = INT64_MAX / 4;
+        if(flux_every_step) { c.lag = 1; c.avg_sample = 1; }
         j.cfg = cfg2(c, nx, ny);
     }
     Batch b(lattice, precision, ctx->stream, 2, false);

@@ -14,23 +14,34 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
 void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDesc *insts,
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
 void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s);
+void kern_reduce(int nwork, const RedWork *work, const InstState *st, const Partial *part,
+                 RPartial *out, hipStream_t s);
 void kern_monitor(int ninst, const InstDesc *insts, const InstCfg *cfgs, InstState *st,
-                  const Partial *lpart, const Partial *npart, hipStream_t s);
+                  const RPartial *lred, const RPartial *nred, hipStream_t s);
 
 // One voxel structure resident on the device: the segment table.
 struct Geometry {
     int dim = 3, nx = 0, ny = 0, nz = 1, nseg = 0;
     int64_t nrows = 0, nunits = 0, n_fluid = 0, nn = 0;
     SegInfo *d_seg = nullptr;
+    void *d_links = nullptr;         // LinkRec<Q-1> per segment (neighbour-step links)
+    uint32_t *d_rowspec = nullptr;   // wrap-lane neighbour index per row and direction
+    size_t links_bytes = 0, rowspec_bytes = 0;
     std::vector<uint8_t> solid;      // host copy, x fastest
     int refs = 0;                    // jobs using it
     ~Geometry();
-    size_t device_bytes() const { return (size_t)nunits * sizeof(SegInfo); }
+    size_t device_bytes() const
+    {
+        return (size_t)nunits * sizeof(SegInfo) + links_bytes + rowspec_bytes;
+    }
 };
 
 // builds the segment table from a mask (x fastest; 1 = solid) and uploads it.
 // Returns an empty string or the error.
 std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz);
+// host-only check of the neighbour link tables (no device memory); *bad =
+// number of (cell, direction) pairs whose derived neighbour is wrong
+std::string geometry_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64_t *bad);
 
 // One forced solve.
 struct Job {
@@ -105,6 +116,7 @@ public:
 
 private:
     std::string rebuild();
+    void launch_step(bool local);
     bool wants_rebuild() const;
     int64_t work_items(const Job &j) const;
     void release_graph();
@@ -123,6 +135,10 @@ private:
     LocalWork *d_lwork_ = nullptr;
     NbrWork *d_nwork_ = nullptr;
     Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
+    RedWork *d_lredw_ = nullptr, *d_nredw_ = nullptr;     // second-level reduction items
+    RPartial *d_lred_ = nullptr, *d_nred_ = nullptr;
+    int n_lred_ = 0, n_nred_ = 0;
+    size_t cap_lredw_ = 0, cap_nredw_ = 0, cap_lred_ = 0, cap_nred_ = 0;
     InstState *h_state_ = nullptr;       // pinned mirror
     int n_inst_ = 0, n_lwork_ = 0, n_nwork_ = 0;
     int64_t work_total_ = 0, work_dead_ = 0;     // launched items, items of retired jobs

@@ -21,9 +21,85 @@ static std::string hip_err(const char *what, hipError_t e)
 Geometry::~Geometry()
 {
     if(d_seg) (void)hipFree(d_seg);
+    if(d_links) (void)hipFree(d_links);
+    if(d_rowspec) (void)hipFree(d_rowspec);
 }
 
-std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz)
+// The neighbour links of every segment (LinkRec): which lanes' neighbours
+// along each direction are fluid and where they sit in the compact order.
+template<class L>
+static void build_links(const Geometry &g, const std::vector<SegInfo> &h,
+                        std::vector<LinkRec<L::Q - 1>> &links, std::vector<uint32_t> &rowspec)
+{
+    constexpr int NK = L::Q - 1;
+    const int nseg = g.nseg, lw = (g.nx - 1) & 63;
+    links.resize((size_t)g.nunits);
+    rowspec.assign((size_t)g.nrows * NK, 0u);
+    auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+    for(int64_t row = 0; row < g.nrows; row++) {
+        const int y = L::DIM == 3 ? (int)(row % g.ny) : (int)row;
+        const int z = L::DIM == 3 ? (int)(row / g.ny) : 0;
+        for(int sg = 0; sg < nseg; sg++) {
+            LinkRec<NK> &R = links[(size_t)(row * nseg + sg)];
+            for(int k = 1; k < L::Q; k++) {
+                const int dx = -L::ex(k), dy = -L::ey(k), dz = -L::ez(k);
+                const int yy = wrap(y + dy, g.ny);
+                const int64_t rr = L::DIM == 3 ? (int64_t)wrap(z + dz, g.nz) * g.ny + yy : yy;
+                const SegInfo &c = h[(size_t)(rr * nseg + sg)];
+                uint64_t M;
+                uint32_t base;
+                if(dx == 0) {
+                    M = c.mask;
+                    base = c.start;
+                } else if(dx < 0) {
+                    // x - 1: this segment shifted up one lane; lane 0 takes the
+                    // previous segment's last cell (the row's x = nx - 1 at sg = 0)
+                    const int sp = sg > 0 ? sg - 1 : nseg - 1;
+                    const SegInfo &pv = h[(size_t)(rr * nseg + sp)];
+                    const uint64_t prevbit = sg > 0 ? (pv.mask >> 63) & 1ull : (pv.mask >> lw) & 1ull;
+                    M = (c.mask << 1) | prevbit;
+                    base = c.start - (uint32_t)prevbit;
+                    if(sg == 0)
+                        rowspec[(size_t)row * NK + k - 1] =
+                            pv.start + (uint32_t)__builtin_popcountll(pv.mask & ((1ull << lw) - 1ull));
+                } else {
+                    // x + 1: shifted down one lane; the segment's last lane takes
+                    // the next segment's first cell (the row's x = 0 at the end)
+                    const int ls = sg < nseg - 1 ? 63 : lw;
+                    const int sn = sg < nseg - 1 ? sg + 1 : 0;
+                    const SegInfo &nx_ = h[(size_t)(rr * nseg + sn)];
+                    M = ((c.mask >> 1) & ~(1ull << ls)) | ((nx_.mask & 1ull) << ls);
+                    base = c.start + (uint32_t)(c.mask & 1ull);
+                    if(sg == nseg - 1) rowspec[(size_t)row * NK + k - 1] = nx_.start;
+                }
+                R.M[k - 1] = M;
+                R.base[k - 1] = base;
+            }
+        }
+    }
+}
+
+template<class L>
+static std::string upload_links(Geometry &g, const std::vector<SegInfo> &h)
+{
+    std::vector<LinkRec<L::Q - 1>> links;
+    std::vector<uint32_t> rowspec;
+    build_links<L>(g, h, links, rowspec);
+    g.links_bytes = links.size() * sizeof(links[0]);
+    g.rowspec_bytes = rowspec.size() * sizeof(uint32_t);
+    hipError_t e = hipMalloc(&g.d_links, g.links_bytes);
+    if(e != hipSuccess) { g.d_links = nullptr; return std::string("hipMalloc(links): ") + hipGetErrorString(e); }
+    e = hipMalloc((void **)&g.d_rowspec, g.rowspec_bytes);
+    if(e != hipSuccess) { g.d_rowspec = nullptr; return std::string("hipMalloc(links): ") + hipGetErrorString(e); }
+    e = hipMemcpy(g.d_links, links.data(), g.links_bytes, hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(g.d_rowspec, rowspec.data(), g.rowspec_bytes, hipMemcpyHostToDevice);
+    if(e != hipSuccess) return std::string("hipMemcpy(links): ") + hipGetErrorString(e);
+    return "";
+}
+
+// host half of geometry_build: dimensions, n_fluid and the segment table
+static std::string build_segments(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz,
+                                  std::vector<SegInfo> &h)
 {
     if(nx <= 0 || ny <= 0 || (dim == 3 && nz <= 0)) return "bad dimensions";
     g.dim = dim;
@@ -32,8 +108,8 @@ std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, i
     g.nrows = (int64_t)g.ny * g.nz;
     g.nunits = g.nrows * g.nseg;
     g.nn = (int64_t)nx * g.nrows;
-    if(g.nunits > INT32_MAX) return "lattice too large";
-    std::vector<SegInfo> h((size_t)g.nunits);
+    if(g.nunits > INT32_MAX || g.nseg > INT16_MAX) return "lattice too large";
+    h.assign((size_t)g.nunits, SegInfo{});
     uint64_t count = 0;
     for(int64_t row = 0; row < g.nrows; row++) {
         const uint8_t *r = solid + row * nx;
@@ -54,11 +130,75 @@ std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, i
     if(count == 0) return "structure has no pore space";
     if(count >= (1ull << 32)) return "too many fluid cells for 32-bit indices";
     g.n_fluid = (int64_t)count;
+    return "";
+}
+
+std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz)
+{
+    std::vector<SegInfo> h;
+    std::string err = build_segments(g, solid, dim, nx, ny, nz, h);
+    if(!err.empty()) return err;
     hipError_t e = hipMalloc(&g.d_seg, h.size() * sizeof(SegInfo));
     if(e != hipSuccess) { g.d_seg = nullptr; return hip_err("hipMalloc(segments)", e); }
     e = hipMemcpy(g.d_seg, h.data(), h.size() * sizeof(SegInfo), hipMemcpyHostToDevice);
     if(e != hipSuccess) return hip_err("hipMemcpy(segments)", e);
+    std::string le = dim == 3 ? upload_links<D3Q19>(g, h) : upload_links<D2Q9>(g, h);
+    if(!le.empty()) return le;
     g.solid.assign(solid, solid + g.nn);
+    return "";
+}
+
+// The neighbour index the step kernel derives from the link tables, for
+// every fluid cell and direction, against a direct lookup on the dense
+// lattice (periodic neighbour, compact index by counting fluid cells).
+template<class L>
+static int64_t verify_links(const Geometry &g, const uint8_t *solid, const std::vector<SegInfo> &h)
+{
+    constexpr int NK = L::Q - 1;
+    std::vector<LinkRec<NK>> links;
+    std::vector<uint32_t> rowspec;
+    build_links<L>(g, h, links, rowspec);
+    std::vector<int64_t> cidx((size_t)g.nn, -1);
+    int64_t c = 0;
+    for(int64_t n = 0; n < g.nn; n++) if(solid[n] == 0) cidx[(size_t)n] = c++;
+    int64_t bad = 0;
+    const int lw = (g.nx - 1) & 63;
+    for(int64_t row = 0; row < g.nrows; row++) {
+        const int y = L::DIM == 3 ? (int)(row % g.ny) : (int)row;
+        const int z = L::DIM == 3 ? (int)(row / g.ny) : 0;
+        for(int sg = 0; sg < g.nseg; sg++) {
+            const LinkRec<NK> &R = links[(size_t)(row * g.nseg + sg)];
+            const SegInfo &own = h[(size_t)(row * g.nseg + sg)];
+            for(int lane = 0; lane < 64; lane++) {
+                if(!((own.mask >> lane) & 1ull)) continue;
+                const int x = sg * 64 + lane;
+                const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+                for(int k = 1; k < L::Q; k++) {
+                    const uint64_t M = R.M[k - 1];
+                    uint32_t nbi = R.base[k - 1] + (uint32_t)__builtin_popcountll(M & below);
+                    if(L::ex(k) > 0 && sg == 0 && lane == 0) nbi = rowspec[(size_t)row * NK + k - 1];
+                    if(L::ex(k) < 0 && sg == g.nseg - 1 && lane == lw) nbi = rowspec[(size_t)row * NK + k - 1];
+                    const bool fluid = (M >> lane) & 1ull;
+                    const int xx = (x - L::ex(k) + g.nx) % g.nx;
+                    const int yy = (y - L::ey(k) + g.ny) % g.ny;
+                    const int zz = L::DIM == 3 ? (z - L::ez(k) + g.nz) % g.nz : 0;
+                    const int64_t dn = ((int64_t)zz * g.ny + yy) * g.nx + xx;
+                    const bool want_fluid = solid[(size_t)dn] == 0;
+                    if(fluid != want_fluid || (fluid && (int64_t)nbi != cidx[(size_t)dn])) bad++;
+                }
+            }
+        }
+    }
+    return bad;
+}
+
+std::string geometry_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64_t *bad)
+{
+    Geometry g;
+    std::vector<SegInfo> h;
+    std::string err = build_segments(g, solid, dim, nx, ny, nz, h);
+    if(!err.empty()) return err;
+    *bad = dim == 3 ? verify_links<D3Q19>(g, solid, h) : verify_links<D2Q9>(g, solid, h);
     return "";
 }
 
@@ -102,6 +242,8 @@ static InstDesc make_desc(const Job &j)
     std::memset(&D, 0, sizeof(D));
     const Geometry &g = *j.geo;
     D.seg = g.d_seg;
+    D.links = g.d_links;
+    D.rowspec = g.d_rowspec;
     D.f = j.d_f;
     D.vel[0] = j.d_vel[0];
     D.vel[1] = j.d_vel[1];
@@ -138,6 +280,10 @@ Batch::~Batch()
     (void)hipFree(d_nwork_);
     (void)hipFree(d_lpart_);
     (void)hipFree(d_npart_);
+    (void)hipFree(d_lredw_);
+    (void)hipFree(d_nredw_);
+    (void)hipFree(d_lred_);
+    (void)hipFree(d_nred_);
     if(h_state_) (void)hipHostFree(h_state_);
 }
 
@@ -238,25 +384,47 @@ std::string Batch::rebuild()
         desc[i].lp1 = (int32_t)lw.size();
         desc[i].np0 = (int32_t)nw.size();
         {
-            // z-slabs of nbr_waves rows; inside a slab y then segment vary fastest
+            // slabs of nbr_waves rows stacked along z (2D: y); inside a slab
+            // y then the segment vary fastest
             const int64_t nwv = tune_.nbr_waves;
             const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
-            const int64_t rows_per_plane = g.dim == 3 ? g.ny : 1;
-            const int64_t ustride = rows_per_plane * g.nseg;
-            for(int64_t z0 = 0; z0 < planes; z0 += nwv) {
-                const int32_t nu = (int32_t)std::min<int64_t>(nwv, planes - z0);
-                for(int64_t u = 0; u < ustride; u++) {
-                    NbrWork w;
-                    w.inst = i;
-                    w.nunit = nu;
-                    w.first = (int32_t)(z0 * ustride + u);
-                    w.ustride = (int32_t)ustride;
-                    nw.push_back(w);
-                }
+            const int64_t ys = g.dim == 3 ? g.ny : 1;
+            for(int64_t p0 = 0; p0 < planes; p0 += nwv) {
+                const int nu = (int)std::min<int64_t>(nwv, planes - p0);
+                for(int64_t y = 0; y < ys; y++)
+                    for(int sg = 0; sg < g.nseg; sg++) {
+                        NbrWork w;
+                        w.inst = i;
+                        w.nunit = (int16_t)nu;
+                        w.seg = (int16_t)sg;
+                        w.y = (int32_t)y;
+                        w.p0 = (int32_t)p0;
+                        nw.push_back(w);
+                    }
             }
         }
         desc[i].np1 = (int32_t)nw.size();
     }
+    // second-level reduction items over each instance's partial ranges
+    std::vector<RedWork> lr, nr;
+    for(int i = 0; i < n_inst_; i++) {
+        auto split = [&](std::vector<RedWork> &out, int32_t p0, int32_t p1, int32_t &r0, int32_t &r1) {
+            r0 = (int32_t)out.size();
+            for(int32_t p = p0; p < p1; p += REDUCE_SPAN) {
+                RedWork r;
+                r.inst = i;
+                r.p0 = p;
+                r.p1 = std::min<int32_t>(p1, p + REDUCE_SPAN);
+                r.pad = 0;
+                out.push_back(r);
+            }
+            r1 = (int32_t)out.size();
+        };
+        split(lr, desc[i].lp0, desc[i].lp1, desc[i].rl0, desc[i].rl1);
+        split(nr, desc[i].np0, desc[i].np1, desc[i].rn0, desc[i].rn1);
+    }
+    n_lred_ = (int)lr.size();
+    n_nred_ = (int)nr.size();
     if(lw.size() > INT32_MAX || nw.size() > INT32_MAX) return "batch too large";
     n_lwork_ = (int)lw.size();
     n_nwork_ = (int)nw.size();
@@ -279,11 +447,18 @@ std::string Batch::rebuild()
     if((e = ensure(d_nwork_, cap_nwork_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
     if((e = ensure(d_lpart_, cap_lpart_, std::max<size_t>(1, lw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
     if((e = ensure(d_npart_, cap_npart_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
+    if((e = ensure(d_lredw_, cap_lredw_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
+       (e = ensure(d_nredw_, cap_nredw_, std::max<size_t>(1, nr.size()))) != hipSuccess ||
+       (e = ensure(d_lred_, cap_lred_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
+       (e = ensure(d_nred_, cap_nred_, std::max<size_t>(1, nr.size()))) != hipSuccess)
+        return hip_err("hipMalloc(reduction)", e);
     e = hipMemcpy(d_inst_, desc.data(), desc.size() * sizeof(InstDesc), hipMemcpyHostToDevice);
     if(e == hipSuccess) e = hipMemcpy(d_cfg_, cfg.data(), cfg.size() * sizeof(InstCfg), hipMemcpyHostToDevice);
     if(e == hipSuccess) e = hipMemcpy(d_state_, state.data(), state.size() * sizeof(InstState), hipMemcpyHostToDevice);
     if(e == hipSuccess && !lw.empty()) e = hipMemcpy(d_lwork_, lw.data(), lw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !nw.empty()) e = hipMemcpy(d_nwork_, nw.data(), nw.size() * sizeof(NbrWork), hipMemcpyHostToDevice);
+    if(e == hipSuccess && !lr.empty()) e = hipMemcpy(d_lredw_, lr.data(), lr.size() * sizeof(RedWork), hipMemcpyHostToDevice);
+    if(e == hipSuccess && !nr.empty()) e = hipMemcpy(d_nredw_, nr.data(), nr.size() * sizeof(RedWork), hipMemcpyHostToDevice);
     if(e != hipSuccess) return hip_err("hipMemcpy(tables)", e);
     dirty_ = false;
     return "";
@@ -309,10 +484,10 @@ std::string Batch::run_chunk()
     auto launch_all = [&]() {
         for(int t = 0; t < chunk_; t++) {
             const bool local = (t & 1) == 0;     // every live job starts a chunk at an even step count
-            kern_step(lattice_, prec_, local, tune_, d_inst_, d_state_,
-                      local ? (const void *)d_lwork_ : (const void *)d_nwork_,
-                      local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
-            kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lpart_, d_npart_, stream_);
+            launch_step(local);
+            kern_reduce(local ? n_lred_ : n_nred_, local ? d_lredw_ : d_nredw_, d_state_,
+                        local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, stream_);
+            kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lred_, d_nred_, stream_);
         }
     };
     if(use_graph_) {
@@ -363,6 +538,13 @@ int64_t Batch::work_items(const Job &j) const
     return (g.n_fluid + lchunk - 1) / lchunk + (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
 }
 
+void Batch::launch_step(bool local)
+{
+    kern_step(lattice_, prec_, local, tune_, d_inst_, d_state_,
+              local ? (const void *)d_lwork_ : (const void *)d_nwork_,
+              local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
+}
+
 std::string Batch::time_steps(int steps, KernelTimes &kt)
 {
     if(wants_rebuild()) {
@@ -380,11 +562,11 @@ std::string Batch::time_steps(int steps, KernelTimes &kt)
     for(int t = 0; t < steps; t++) {
         const bool local = (t & 1) == 0;
         (void)hipEventRecord(ev[k++], stream_);
-        kern_step(lattice_, prec_, local, tune_, d_inst_, d_state_,
-                  local ? (const void *)d_lwork_ : (const void *)d_nwork_,
-                  local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
+        launch_step(local);
         (void)hipEventRecord(ev[k++], stream_);
-        kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lpart_, d_npart_, stream_);
+        kern_reduce(local ? n_lred_ : n_nred_, local ? d_lredw_ : d_nredw_, d_state_,
+                    local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, stream_);
+        kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lred_, d_nred_, stream_);
         (void)hipEventRecord(ev[k++], stream_);
         (void)hipEventRecord(ev[k++], stream_);
     }

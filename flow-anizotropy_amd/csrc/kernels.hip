@@ -300,22 +300,13 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
-// The 9 rows (dy, dz) in {-1,0,1}^2 around a row, numbered (dy+1)*3 + (dz+1);
-// which of them a lattice reaches and whether it needs the previous / next
-// segment there (a link with dx = -1 / +1).
-template<class L> __device__ __forceinline__ constexpr int row_of(int k)
+// a when bit `lane` of the wave-uniform mask M is set, else b: one
+// v_cndmask with M as the lane-select operand
+__device__ __forceinline__ uint32_t lane_select(uint64_t M, uint32_t a, uint32_t b)
 {
-    return (-L::ey(k) + 1) * 3 + (-L::ez(k) + 1);
-}
-template<class L> __device__ __forceinline__ constexpr bool row_dx(int r, int dx)
-{
-    for(int k = 1; k < L::Q; k++)
-        if(row_of<L>(k) == r && -L::ex(k) == dx) return true;
-    return false;
-}
-template<class L> __device__ __forceinline__ constexpr bool row_used(int r)
-{
-    return row_dx<L>(r, -1) || row_dx<L>(r, 0) || row_dx<L>(r, 1);
+    uint32_t r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(M));
+    return r;
 }
 
 template<class L, int P, bool NT>
@@ -325,6 +316,7 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 {
     using T = typename Store<L, P>::T;
     constexpr int Q = L::Q;
+    constexpr int NK = Q - 1;
     const int wave = uni(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const ItemRange R = my_items(nwork);
@@ -336,19 +328,14 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         const InstDesc &D = insts[w.inst];
         gptr<T> f = as_global<T>(D.f);
         const uint32_t stride = (uint32_t)D.stride;    // Q * stride < 2^32 (checked on the host)
-        const int nx = D.nx, ny = D.ny, nz = D.nz, nseg = D.nseg;
-        const int lw = (nx - 1) & 63;                  // lane of x = nx - 1 in the last segment
+        const int ny = D.ny, nseg = D.nseg;
         cptr<SegInfo> seg = as_const<SegInfo>(D.seg);
 
         const bool wave_act = wave < w.nunit;
-        const int unit = uni(w.first + (wave_act ? wave : 0) * w.ustride);
-        const int row = unit / nseg;
-        const int sg = unit - row * nseg;
-        const int y = (L::DIM == 3) ? row % ny : row;
-        const int z = (L::DIM == 3) ? row / ny : 0;
-        const int sp = sg > 0 ? sg - 1 : nseg - 1;     // previous / next segment (periodic)
-        const int sn = sg < nseg - 1 ? sg + 1 : 0;
-        const int ls = sg < nseg - 1 ? 63 : lw;        // lane whose x+1 leaves the segment
+        const int p = uni(w.p0 + (wave_act ? wave : 0));
+        const int sg = w.seg;
+        const int row = (L::DIM == 3) ? p * ny + w.y : p;
+        const int unit = row * nseg + sg;
 
         const uint64_t m0 = uni64(seg[unit].mask);
         const uint32_t i = lane_rank_plus(m0, uni(seg[unit].start));
@@ -356,51 +343,33 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 
         double u[3] = {0.0, 0.0, 0.0};
         if(act) {
-            // segment data of the reachable neighbour rows, loaded once
-            uint64_t mc[9], mp[9], mn[9];
-            uint32_t sc[9], spv[9], snv[9];
-#pragma unroll
-            for(int r = 0; r < 9; r++) {
-                mc[r] = mp[r] = mn[r] = 0; sc[r] = spv[r] = snv[r] = 0;
-                if(!row_used<L>(r)) continue;
-                const int dy = r / 3 - 1, dz = r % 3 - 1;
-                int yy = y + dy; yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
-                int zz = z;
-                if constexpr(L::DIM == 3) { zz = z + dz; zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz); }
-                const int rb = uni((zz * ny + yy) * nseg);
-                mc[r] = uni64(seg[rb + sg].mask);
-                sc[r] = uni(seg[rb + sg].start);
-                if(row_dx<L>(r, -1)) { mp[r] = uni64(seg[rb + sp].mask); spv[r] = uni(seg[rb + sp].start); }
-                if(row_dx<L>(r, 1))  { mn[r] = uni64(seg[rb + sn].mask); snv[r] = uni(seg[rb + sn].start); }
-            }
+            cptr<LinkRec<NK>> lr = as_const<LinkRec<NK>>(D.links) + unit;
+            cptr<uint32_t> rs = as_const<uint32_t>(D.rowspec) + (int64_t)row * NK;
+            // the lanes whose x-neighbour wraps around the row
+            const bool wrapL = sg == 0 && lane == 0;
+            const bool wrapR = sg == nseg - 1 && lane == ((D.nx - 1) & 63);
             // off[k]: the location population k is read from this step and the
             // location population opp(k) is written to: f[opp k][cell - e_k]
             // when that neighbour is fluid, else (halfway bounce-back) f[k][cell]
+            // the segment's whole link record and the row's wrap indices, in
+            // scalar registers before any use (the loads overlap)
+            uint64_t Mk[NK];
+            uint32_t Bk[NK], Sk[NK];
+#pragma unroll
+            for(int k = 0; k < NK; k++) {
+                Mk[k] = lr->M[k];
+                Bk[k] = lr->base[k];
+                Sk[k] = L::ex(k + 1) != 0 ? rs[k] : 0u;
+            }
             uint32_t off[Q];
             off[0] = i;
 #pragma unroll
             for(int k = 1; k < Q; k++) {
-                const int r = row_of<L>(k), dx = -L::ex(k);
                 const uint32_t add = (uint32_t)L::opp(k) * stride;
-                const uint64_t m = mc[r];
-                uint64_t M;
-                uint32_t nbi;
-                if(dx == 0) {
-                    M = m;
-                    nbi = lane_rank_plus(m, sc[r] + add);
-                } else if(dx < 0) {
-                    const uint64_t prevbit = sg > 0 ? (mp[r] >> 63) : ((mp[r] >> lw) & 1ull);
-                    M = (m << 1) | prevbit;
-                    const uint32_t l0 = sg > 0 ? sc[r] - 1u
-                                               : spv[r] + (uint32_t)__popcll(mp[r] & ((1ull << lw) - 1ull));
-                    const uint32_t gen = lane_rank_plus(m << 1, sc[r] + add);
-                    nbi = lane == 0 ? l0 + add : gen;
-                } else {
-                    M = ((m >> 1) & ~(1ull << ls)) | ((mn[r] & 1ull) << ls);
-                    const uint32_t gen = lane_rank_plus(m >> 1, sc[r] + (uint32_t)(m & 1ull) + add);
-                    nbi = lane == ls ? snv[r] + add : gen;
-                }
-                off[k] = ((M >> lane) & 1ull) ? nbi : i + (uint32_t)k * stride;
+                uint32_t nbi = lane_rank_plus(Mk[k - 1], Bk[k - 1] + add);
+                if(L::ex(k) > 0) nbi = wrapL ? Sk[k - 1] + add : nbi;
+                if(L::ex(k) < 0) nbi = wrapR ? Sk[k - 1] + add : nbi;
+                off[k] = lane_select(Mk[k - 1], nbi, i + (uint32_t)k * stride);
             }
             T v[Q];
 #pragma unroll
@@ -424,10 +393,63 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 // monitor: exact flux of every instance that produced one this step, then the
 // convergence state machine.  One workgroup per instance.
 // ---------------------------------------------------------------------------
+// sum of 128-bit values across a 256-thread workgroup (result in thread 0)
+__device__ __forceinline__ void block_sum_i128(I128 (&s)[3], int &bad)
+{
+    __shared__ I128 acc[3][256];
+    __shared__ int accbad[256];
+    for(int k = 0; k < 3; k++) acc[k][threadIdx.x] = s[k];
+    accbad[threadIdx.x] = bad;
+    __syncthreads();
+    for(int h = blockDim.x >> 1; h > 0; h >>= 1) {
+        if((int)threadIdx.x < h) {
+            for(int k = 0; k < 3; k++)
+                acc[k][threadIdx.x] = i128_add(acc[k][threadIdx.x], acc[k][threadIdx.x + h]);
+            accbad[threadIdx.x] |= accbad[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    for(int k = 0; k < 3; k++) s[k] = acc[k][0];
+    bad = accbad[0];
+}
+
+// ---------------------------------------------------------------------------
+// first reduction level of the exact flux: every workgroup folds up to
+// REDUCE_SPAN step-kernel partials of one instance into 128-bit sums, so the
+// per-instance monitor adds a few hundred values, not a lattice's worth.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_reduce(const RedWork *__restrict__ work, const InstState *__restrict__ st,
+         const Partial *__restrict__ part, RPartial *__restrict__ out)
+{
+    const RedWork w = work[blockIdx.x];
+    if(st[w.inst].phase >= PH_DONE || !st[w.inst].flux_next) return;
+    I128 s[3] = {i128_zero(), i128_zero(), i128_zero()};
+    int bad = 0;
+    for(int p = w.p0 + (int)threadIdx.x; p < w.p1; p += blockDim.x) {
+        const Partial pp = part[p];
+        for(int k = 0; k < 3; k++)
+            s[k] = i128_add(s[k], i128_from_chunks(pp.c[k][0], pp.c[k][1], pp.c[k][2]));
+        bad |= pp.bad;
+    }
+    block_sum_i128(s, bad);
+    if(threadIdx.x == 0) {
+        RPartial r;
+        for(int k = 0; k < 3; k++) r.s[k] = s[k];
+        r.bad = bad;
+        r.pad[0] = r.pad[1] = r.pad[2] = 0;
+        out[blockIdx.x] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// monitor: exact flux of every instance that produced one this step, then the
+// convergence state machine.  One workgroup per instance.
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_monitor(const InstDesc *__restrict__ insts, const InstCfg *__restrict__ cfgs,
-          InstState *__restrict__ st, const Partial *__restrict__ lpart,
-          const Partial *__restrict__ npart)
+          InstState *__restrict__ st, const RPartial *__restrict__ lred,
+          const RPartial *__restrict__ nred)
 {
     const int inst = blockIdx.x;
     // thread 0 rewrites st[inst] at the end: read it once, through LDS
@@ -445,36 +467,23 @@ k_monitor(const InstDesc *__restrict__ insts, const InstCfg *__restrict__ cfgs,
     const int64_t t = sh_t;
     const InstDesc &D = insts[inst];
 
-    __shared__ I128 acc[3][256];
-    __shared__ int accbad[256];
     double q[3] = {0.0, 0.0, 0.0};
     if(flux) {
         const bool local = (t & 1) != 0;
-        const Partial *P = local ? lpart : npart;
-        const int p0 = local ? D.lp0 : D.np0, p1 = local ? D.lp1 : D.np1;
+        const RPartial *P = local ? lred : nred;
+        const int p0 = local ? D.rl0 : D.rn0, p1 = local ? D.rl1 : D.rn1;
         I128 s[3] = {i128_zero(), i128_zero(), i128_zero()};
         int bad = 0;
         for(int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
-            const Partial pp = P[p];
-            for(int k = 0; k < 3; k++)
-                s[k] = i128_add(s[k], i128_from_chunks(pp.c[k][0], pp.c[k][1], pp.c[k][2]));
+            const RPartial &pp = P[p];
+            for(int k = 0; k < 3; k++) s[k] = i128_add(s[k], pp.s[k]);
             bad |= pp.bad;
         }
-        for(int k = 0; k < 3; k++) acc[k][threadIdx.x] = s[k];
-        accbad[threadIdx.x] = bad;
-        __syncthreads();
-        for(int h = blockDim.x >> 1; h > 0; h >>= 1) {
-            if((int)threadIdx.x < h) {
-                for(int k = 0; k < 3; k++)
-                    acc[k][threadIdx.x] = i128_add(acc[k][threadIdx.x], acc[k][threadIdx.x + h]);
-                accbad[threadIdx.x] |= accbad[threadIdx.x + h];
-            }
-            __syncthreads();
-        }
+        block_sum_i128(s, bad);
         if(threadIdx.x == 0) {
             const InstCfg &c = cfgs[inst];
-            for(int k = 0; k < 3; k++) q[k] = i128_to_double_scaled(acc[k][0]) / c.denom[k];
-            if(accbad[0]) q[0] = q[1] = q[2] = __builtin_nan("");
+            for(int k = 0; k < 3; k++) q[k] = i128_to_double_scaled(s[k]) / c.denom[k];
+            if(bad) q[0] = q[1] = q[2] = __builtin_nan("");
         }
     }
     if(threadIdx.x == 0) {
@@ -604,11 +613,18 @@ void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDe
     }
 }
 
+void kern_reduce(int nwork, const RedWork *work, const InstState *st, const Partial *part,
+                 RPartial *out, hipStream_t s)
+{
+    if(nwork <= 0) return;
+    hipLaunchKernelGGL(k_reduce, dim3(nwork), dim3(256), 0, s, work, st, part, out);
+}
+
 void kern_monitor(int ninst, const InstDesc *insts, const InstCfg *cfgs, InstState *st,
-                  const Partial *lpart, const Partial *npart, hipStream_t s)
+                  const RPartial *lred, const RPartial *nred, hipStream_t s)
 {
     if(ninst <= 0) return;
-    hipLaunchKernelGGL(k_monitor, dim3(ninst), dim3(256), 0, s, insts, cfgs, st, lpart, npart);
+    hipLaunchKernelGGL(k_monitor, dim3(ninst), dim3(256), 0, s, insts, cfgs, st, lred, nred);
 }
 
 } // namespace fp

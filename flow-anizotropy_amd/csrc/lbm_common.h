@@ -162,6 +162,19 @@ struct SegInfo {
     uint32_t pad;
 };
 
+// Neighbour links of one 64-cell row segment, built once per structure.  For
+// every direction k >= 1 (entry k - 1), the cell of lane l reads population
+// opp(k) at the cell - e_k:
+//   M[k-1]     bit l set when that cell is fluid (else the link bounces back),
+//   base[k-1]  that cell's compact index is base + popcount(M & ((1 << l) - 1)),
+// except for one lane at the periodic x-wrap (the first segment's lane 0 when
+// e_k.x = +1, the last segment's lane of x = nx - 1 when e_k.x = -1), whose
+// neighbour's index is kept per row: rowspec[row * (Q - 1) + k - 1].
+template<int NK> struct LinkRec {
+    uint64_t M[NK];
+    uint32_t base[NK];
+};
+
 enum Prec : int {
     PREC_F64 = 0,       // f in double, the reference's arithmetic
     PREC_F32 = 1,       // f - w in float (shifted), FP64 flux sums
@@ -170,6 +183,8 @@ enum Prec : int {
 
 struct InstDesc {
     const SegInfo *seg;     // [nrows * nseg]
+    const void *links;      // LinkRec<Q-1>[nrows * nseg]
+    const uint32_t *rowspec;   // [nrows * (Q-1)]
     void    *f;             // populations f[k * stride + i]
     float   *vel[2];        // compact velocity fields (DIM floats per cell), or null
     int64_t  stride;        // elements per population array
@@ -181,8 +196,10 @@ struct InstDesc {
     int      prec;
     // FP32 D3Q19 per-direction force terms 4.5 w_k (e_k.F) and 1.5 w_k (e_k.F)
     float    g45[19], h15[19];
-    // partial-sum slots of this instance in the local / neighbour kernels
+    // partial-sum slots of this instance in the local / neighbour kernels,
+    // and its second-level slots for each
     int32_t  lp0, lp1, np0, np1;
+    int32_t  rl0, rl1, rn0, rn1;
 };
 
 // convergence state of one solve, advanced on the device after every step.
@@ -402,6 +419,20 @@ struct Partial {
     int32_t pad;
 };
 
+// second level: up to REDUCE_SPAN step-kernel partials of one instance
+// folded into 128-bit integers by one workgroup
+constexpr int REDUCE_SPAN = 1024;
+struct RPartial {
+    I128    s[3];
+    int32_t bad;
+    int32_t pad[3];
+};
+struct RedWork {
+    int32_t inst;
+    int32_t p0, p1;           // step-kernel partials [p0, p1)
+    int32_t pad;
+};
+
 // launch shapes of the step kernels (chosen by the autotuner)
 struct Tuning {
     int local_cpt = 1;      // cells per thread in the local kernel: 1, 2 or 4
@@ -423,11 +454,15 @@ struct LocalWork {
 // segment) units of one instance, one wavefront each, at first + w * ustride
 // (ustride = one row up in z, so the rows that share cache lines through the
 // z-links are processed together)
+// One neighbour-step work item: nunit row segments stacked along the slowest
+// axis, wave w taking the one at plane p0 + w.  3D: rows (y, z = p0 + w);
+// 2D: rows y = p0 + w.  seg is the 64-cell segment of the row.
 struct NbrWork {
     int32_t inst;
-    int32_t nunit;
-    int32_t first;
-    int32_t ustride;
+    int16_t nunit;
+    int16_t seg;
+    int32_t y;
+    int32_t p0;
 };
 
 } // namespace fp

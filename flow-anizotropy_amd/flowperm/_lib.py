@@ -128,6 +128,7 @@ SIGNATURES = {
     "fp_solve_dir3d": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int,
                                  C.POINTER(Config3D), _dp, C.POINTER(SolveOut), _fp]),
     "fp_jacobi3": (None, [_dp, _dp, _dp]),
+    "fp_verify_links": (C.c_int, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
     "fp_config2d_default": (None, [C.POINTER(Config2D)]),
     "fp_solve2d": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int,
                              C.POINTER(Config2D), C.POINTER(Result2D), _fp]),
@@ -145,7 +146,8 @@ SIGNATURES = {
                               C.POINTER(C.c_double)]),
     "fp_batch_set_chunk": (None, [C.c_void_p, C.c_int]),
     "fp_bench_steps": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 _dp, C.c_int, C.c_int, C.POINTER(Tuning), C.POINTER(BenchOut)]),
+                                 _dp, C.c_int, C.c_int, C.POINTER(Tuning), C.c_int,
+                                 C.POINTER(BenchOut)]),
     "fp_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "fp_autotune": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.c_int, C.POINTER(Tuning), _dp, _dp]),

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 2
+#define FP_ABI_VERSION 3
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -141,6 +141,14 @@ int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
  * force F (lattice units).  vel: NULL or NN*3 floats. */
 int fp_solve_dir3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
                    const fp_config3d *cfg, const double F[3], fp_solve_out *out, float *vel);
+
+/* Self-check of the neighbour tables the step kernels use (no reference
+ * counterpart; host only, no device work): for every fluid cell and lattice
+ * direction, the neighbour index the kernels derive from the per-segment link
+ * records against a direct lookup on the dense periodic lattice.  dim 3 =
+ * D3Q19 ([nz][ny][nx]), dim 2 = D2Q9 ([ny][nx], nz ignored).  *mismatches = 0
+ * when every link is right. */
+int fp_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64_t *mismatches);
 
 /* Symmetric 3x3 eigen-decomposition exactly as jacobi3 (:401-443). */
 void fp_jacobi3(const double A[9], double val[3], double vec[9]);
@@ -254,10 +262,12 @@ typedef struct fp_tuning {
 int  fp_set_tuning(fp_ctx *ctx, const fp_tuning *t);
 void fp_get_tuning(fp_ctx *ctx, fp_tuning *t);
 
-/* tune: NULL = the context's shapes */
+/* tune: NULL = the context's shapes.  flux_every_step: 0 = convergence checks
+ * at the default lag (2000 steps), as in the first phase of a solve; 1 = the
+ * exact flux sum on every step, as in the averaging phase. */
 int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
                    int precision, const double F[3], int warmup, int steps,
-                   const fp_tuning *tune, fp_bench_out *out);
+                   const fp_tuning *tune, int flux_every_step, fp_bench_out *out);
 
 /* Autotuner: times every launch shape of the two step kernels on the given
  * lattice (`steps` steps each), stores the fastest in *best and makes it the

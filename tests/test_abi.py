@@ -156,3 +156,28 @@ def test_tune_shapes_are_valid(lib):
 def test_now_is_monotonic(lib):
     t = [lib.fp_now() for _ in range(100)]
     assert all(b >= a for a, b in zip(t, t[1:]))
+
+
+def test_neighbour_link_tables_match_a_dense_lookup(lib):
+    """the per-segment link records the neighbour kernel reads give, for every
+    fluid cell and direction, the compact index of the periodic neighbour at
+    cell - e_k (or mark it solid): random lattices of every width class
+    (x-runs of 1, < 64, 64, > 64 cells) in 3D and 2D"""
+    from flowperm import _lib
+    rng = np.random.default_rng(0)
+    shapes = [(1, 1, 1), (2, 3, 1), (3, 2, 63), (2, 2, 64), (3, 4, 65), (2, 3, 128), (5, 4, 130),
+              (9, 7, 11), (16, 16, 16)]
+    for shp in shapes:
+        for dim in (3, 2):
+            s = shp if dim == 3 else shp[1:]
+            for por in (0.3, 0.7, 1.0):
+                m = (rng.random(s) > por).astype(np.uint8)
+                m.flat[0] = 0
+                bad = C.c_int64(-1)
+                if dim == 3:
+                    nz, ny, nx = s
+                else:
+                    (ny, nx), nz = s, 1
+                st = lib.fp_verify_links(_lib.u8ptr(np.ascontiguousarray(m)), dim, nx, ny, nz,
+                                         C.byref(bad))
+                assert st == 0 and bad.value == 0, (s, dim, por, bad.value)
