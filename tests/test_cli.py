@@ -1,0 +1,123 @@
+"""
+The lbm3d-perm / lbm2d-perm drop-ins (flow-anizotropy_amd/bin/): argument
+handling, exit codes and CSV text of the reference executables
+(LMB3d/lbm3d-perm.cpp:446-630, LMB2d/main.cpp:238-426).  The error paths run
+on the CPU (they fail before any device work); full runs are GPU tests.
+"""
+import csv
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO, assert_printed_close, golden
+
+BIN = REPO / "flow-anizotropy_amd" / "bin"
+
+
+def run(prog, *args, cwd=None):
+    p = subprocess.run([sys.executable, str(BIN / prog), *map(str, args)],
+                       capture_output=True, text=True, cwd=cwd, timeout=600)
+    return p.returncode, p.stdout, p.stderr
+
+
+def test_usage_and_argument_errors(tmp_path):
+    rc, _, err = run("lbm3d-perm")
+    assert rc == 2 and err.startswith("Usage: lbm3d-perm")       # argc < 3 -> 2
+    rc, _, err = run("lbm2d-perm")
+    assert rc == 1 and err.startswith("Usage: lbm2d-perm")       # main.cpp:280
+    rc, _, err = run("lbm3d-perm", "a.raw", "b.csv", "--bogus")
+    assert rc == 2 and err.strip().endswith("lbm3d-perm: unknown option --bogus")
+    rc, _, err = run("lbm3d-perm", "a.raw", "b.csv", "--lag")
+    assert rc == 2 and "lbm3d-perm: missing value for --lag" in err
+    rc, _, err = run("lbm2d-perm", "a.gif", "b.csv", "--lag", "0")
+    assert rc == 2 and "lbm2d-perm: --lag and --avg-sample must be positive" in err
+
+
+def test_structure_errors_are_exit_2(tmp_path):
+    from flowperm import structio
+    rc, _, err = run("lbm3d-perm", tmp_path / "missing.raw", tmp_path / "o.csv", "--quiet")
+    assert rc == 2 and "lbm3d-perm: cannot open" in err
+    structio.save_raw(tmp_path / "solid.raw", np.ones((3, 3, 3), np.uint8))
+    rc, _, err = run("lbm3d-perm", tmp_path / "solid.raw", tmp_path / "o.csv", "--quiet")
+    assert rc == 2 and err.strip() == "lbm3d-perm: structure has no pore space"
+    raw = (tmp_path / "solid.raw").read_bytes()
+    (tmp_path / "trunc.raw").write_bytes(raw[:-2])
+    rc, _, err = run("lbm3d-perm", tmp_path / "trunc.raw", tmp_path / "o.csv", "--quiet")
+    assert rc == 2 and "truncated payload" in err
+    assert not (tmp_path / "o.csv").exists()                      # no row on a hard error
+
+
+def test_numeric_flags_follow_strtol_strtod():
+    from flowperm.cli import _strtod, _strtol
+    assert _strtol("2000") == 2000 and _strtol("12abc") == 12 and _strtol("x") == 0
+    assert _strtol("-5") == -5 and _strtol(" 7") == 7
+    assert _strtod("1e-6") == 1e-6 and _strtod("40") == 40.0 and _strtod("2.5x") == 2.5
+    assert _strtod("abc") == 0.0
+
+
+@pytest.mark.gpu
+def test_lbm3d_row_matches_the_reference_row(tmp_path):
+    """the drop-in on a reference fixture: header text identical, values at
+    the reference's printed precision (1e-10 x trace floor), same steps and
+    exit code; a second run appends a row without a header"""
+    name = "sph16_s0"
+    mask, row, meta = golden(name)
+    argv = meta["argv"][3:]
+    out = tmp_path / "o.csv"
+    rc, stdout, err = run("lbm3d-perm", GOLDEN / f"{name}.raw", out, *argv)
+    print(stdout, err, flush=True)
+    assert rc == meta["returncode"]
+    ref_lines = (GOLDEN / f"{name}.csv").read_text().splitlines()
+    lines = out.read_text().splitlines()
+    assert lines[0] == ref_lines[0]
+    ours = dict(zip(lines[0].split(","), next(csv.reader([lines[1]]))))
+    refd = dict(zip(ref_lines[0].split(","), next(csv.reader([ref_lines[1]]))))
+    trace = abs(float(refd["K_xx"]) + float(refd["K_yy"]) + float(refd["K_zz"]))
+    for k, v in refd.items():
+        if k == "filename":
+            continue
+        if k.startswith(("steps_", "conv_", "nx", "ny", "nz")):
+            assert ours[k] == v, k
+        elif k.startswith(("K_", "k_", "k1", "k2", "k3")):
+            assert_printed_close(float(ours[k]), float(v), 1e-10 * trace, k)
+    rc2, _, _ = run("lbm3d-perm", GOLDEN / f"{name}.raw", out, *argv)
+    assert rc2 == rc and len(out.read_text().splitlines()) == 3
+
+
+@pytest.mark.gpu
+def test_lbm3d_velocity_files(tmp_path):
+    name = "sph12_velocity_s4"
+    _, _, meta = golden(name)
+    argv = [a for a in meta["argv"][3:]]
+    i = argv.index("--velocity")
+    argv[i + 1] = str(tmp_path / "v")
+    rc, _, err = run("lbm3d-perm", GOLDEN / f"{name}.raw", tmp_path / "o.csv", *argv)
+    assert rc == meta["returncode"], err
+    for t in ("fx", "fy", "fz"):
+        ours = (tmp_path / f"v.{t}.vel").read_bytes()
+        ref = (GOLDEN / f"{name}.{t}.vel").read_bytes()
+        assert ours[:16] == ref[:16]
+        a = np.frombuffer(ours, "<f4", offset=16)
+        b = np.frombuffer(ref, "<f4", offset=16)
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+
+
+@pytest.mark.gpu
+def test_lbm2d_row_and_velocity_export(tmp_path):
+    from flowperm import masks, solver, structio
+    m = masks.ellipses((64, 64), 0.6, seed=1)
+    structio.save_gif2d(tmp_path / "m.gif", m)
+    flags = ["--lag", "500", "--min-steps", "1000", "--max-steps", "8000", "--avg-steps", "1000",
+             "--quiet", "--velocity", str(tmp_path / "v")]
+    rc, _, err = run("lbm2d-perm", tmp_path / "m.gif", tmp_path / "o.csv", *flags)
+    assert rc in (0, 3), err
+    lines = (tmp_path / "o.csv").read_text().splitlines()
+    assert lines[0] == ",".join(solver.COLUMNS2D)
+    r = solver.solve_permeability2d(m, lag=500, min_steps=1000, max_steps=8000, avg_steps=1000,
+                                    velocity=True, filename=str(tmp_path / "m.gif"))
+    assert lines[1] + "\n" == solver.csv_row(r, solver.COLUMNS2D)
+    txt = (tmp_path / "v.fx.dat").read_text().split()
+    vals = np.array([float(x) for x in txt]).reshape(64, 64, 2)
+    assert np.allclose(vals, r["velocity"][0], rtol=1e-5, atol=1e-12)
