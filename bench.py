@@ -74,7 +74,8 @@ def bench_steps(ctx, mask, prec, warmup, steps, tune=None, flux=0):
     F = np.array([2.5e-7 * 40, 0.0, 0.0])
     t = None
     if tune is not None:
-        t = _lib.Tuning(tune[0], tune[1], 200, tune[2], tune[3], tune[4])
+        t = _lib.Tuning(tune[0], tune[1], 200, tune[2], tune[3], tune[4],
+                        tune[5] if len(tune) > 5 else 0)
     st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, 3, prec,
                                 _lib.f64ptr(F), warmup, steps, C.byref(t) if t else None,
                                 int(flux), C.byref(out))
@@ -108,7 +109,7 @@ def apply_tuning(ctx, name):
     r = TUNED.get(name)
     if r:
         t = _lib.Tuning(r["local_cpt"], r["nbr_waves"], r["chunk_steps"], r["nontemporal"],
-                        r["local_bpc"], r["nbr_bpc"])
+                        r["local_bpc"], r["nbr_bpc"], r.get("nbr_mode", 0))
         ctx.lib.fp_set_tuning(ctx.handle, C.byref(t))
 
 
@@ -207,6 +208,137 @@ def serving_sample(ctx, n, seed, workers, load=0.8):
     return res
 
 
+def k2_matrix(row):
+    return np.array([[row["k_xx"], row["k_xy"]], [row["k_yx"], row["k_yy"]]])
+
+
+def k2_error(a, b):
+    A, B = k2_matrix(a), k2_matrix(b)
+    return float((np.abs(A - B) / np.maximum(np.abs(B), abs(np.trace(B)))).max())
+
+
+def bench2d(ctx, mask, prec, warmup, steps):
+    from flowperm import _lib
+    ny, nx = mask.shape
+    out = _lib.BenchOut()
+    F = np.array([2.5e-7, 0.0, 0.0])
+    st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(mask), nx, ny, 1, 2, prec, _lib.f64ptr(F),
+                                warmup, steps, None, 0, C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"fp_bench_steps (2D) failed: {st} {ctx.last_error()}")
+    return out
+
+
+def solve_logged(ctx, mask, precision, every=20.0, **cfg):
+    """one whole solve through a request server (the same result as the
+    one-shot solve), with a progress line every `every` seconds"""
+    from flowperm.server import Server
+    lattice = 3 if mask.ndim == 3 else 2
+    apply_tuning(ctx, "f64" if precision == "f64" else "f32")
+    t0 = time.perf_counter()
+    last = t0
+    with Server(lattice, precision, ctx=ctx) as srv:
+        rid = srv.submit(mask, **cfg)
+        while srv.step():
+            if time.perf_counter() - last > every:
+                last = time.perf_counter()
+                st = srv.stats()
+                log(f"  ... {precision} {mask.shape}: {st['steps']} steps, "
+                    f"{st['live_instances']} runs live, {last - t0:.0f} s")
+        row, _ = srv.poll(rid)
+    row["wall_s"] = time.perf_counter() - t0
+    return row
+
+
+def precision_study(ctx, seed, steps):
+    """The single-structure precision study: full solves of the 256^3
+    ellipsoid lattice (D3Q19) and of a 1024^2 ellipse lattice (D2Q9) in FP64
+    and FP32-shifted (2D also in the reference's float arithmetic), with the
+    stepping rates and the per-component K error against FP64."""
+    from flowperm import masks, solver
+    res = {}
+    mask, _ = workload_mask(256, seed)
+    apply_tuning(ctx, "f32")
+    o32 = bench_steps(ctx, mask, 1, 20, steps)
+    apply_tuning(ctx, "f64")
+    o64 = bench_steps(ctx, mask, 0, 20, steps)
+    runs = {}
+    for prec in ("f64", "f32"):
+        runs[prec] = solve_logged(ctx, mask, prec, **SOLVE_CFG)
+        log(f"3D {prec}: steps {[runs[prec][f'steps_{t}'] for t in ('fx', 'fy', 'fz')]} "
+            f"in {runs[prec]['wall_s']:.1f} s")
+    K64, K32 = k_matrix(runs["f64"]), k_matrix(runs["f32"])
+    rel = np.abs(K32 - K64) / np.maximum(np.abs(K64), abs(np.trace(K64)))
+    res["d3q19_256"] = {
+        "porosity": float(1 - mask.mean()), "n_fluid": int((mask == 0).sum()),
+        "mlups": {"f32": o32.n_fluid * o32.steps / o32.total_ms / 1e3,
+                  "f64": o64.n_fluid * o64.steps / o64.total_ms / 1e3},
+        "step_kernel_GBs": {
+            "f32": o32.n_fluid * 152 / (0.5 * (o32.local_ms + o32.nbr_ms) / 1e3) / 1e9,
+            "f64": o64.n_fluid * 304 / (0.5 * (o64.local_ms + o64.nbr_ms) / 1e3) / 1e9},
+        "K_f64": K64.tolist(), "K_f32": K32.tolist(),
+        "rel_error_per_component": rel.tolist(), "max_rel_error": float(rel.max()),
+        "steps": {p: [runs[p][f"steps_{t}"] for t in ("fx", "fy", "fz")] for p in runs},
+        "solve_seconds": {p: runs[p]["wall_s"] for p in runs},
+        "config": "--ff 40 --lag 1000 --eps 1e-6; error = |K32 - K64| / max(|K64|, |tr K64|)"}
+    log(f"3D 256^3: max K error FP32 vs FP64 {rel.max():.2e}")
+
+    rng = np.random.default_rng(seed)
+    por = 0.3 + 0.5 * rng.random()
+    m2 = masks.ellipses((1024, 1024), por, rmin=2.0, rmax=4.0, aspect=3.0, jitter_deg=15.0, seed=seed)
+    rates, rows = {}, {}
+    L0 = 4.0 * 1024 / 256
+    for name, prec in (("f32ref", 2), ("f64", 0), ("f32", 1)):
+        apply_tuning(ctx, "f64" if prec == 0 else "f32")
+        o = bench2d(ctx, m2, prec, 20, steps)
+        rates[name] = {"mlups": o.n_fluid * o.steps / o.total_ms / 1e3,
+                       "step_kernel_GBs": o.n_fluid * o.bytes_per_update /
+                                          (0.5 * (o.local_ms + o.nbr_ms) / 1e3) / 1e9}
+        rows[name] = solve_logged(ctx, m2, name, L0=L0)
+        log(f"2D {name}: {rates[name]['mlups']:.0f} MLUPS, steps "
+            f"{rows[name]['steps_fx']}/{rows[name]['steps_fy']} in {rows[name]['wall_s']:.1f} s")
+    res["d2q9_1024"] = {
+        "porosity": float(1 - m2.mean()), "n_fluid": int((m2 == 0).sum()), "rates": rates,
+        "K": {n: k2_matrix(r).tolist() for n, r in rows.items()},
+        "max_rel_error_vs_f64": {n: k2_error(rows[n], rows["f64"]) for n in ("f32ref", "f32")},
+        "steps": {n: [r["steps_fx"], r["steps_fy"]] for n, r in rows.items()},
+        "solve_seconds": {n: r["wall_s"] for n, r in rows.items()},
+        "config": "reference 2D defaults (eps 1e-3, lag 5000, hold 3, min 10000, avg 20000 "
+                  "sampled every 50); L0 = 4 * 1024 / 256"}
+    log(f"2D 1024^2: max K error vs FP64: {res['d2q9_1024']['max_rel_error_vs_f64']}")
+    return res
+
+
+def large_lattice(ctx, seed, steps, solve):
+    """The 512^3 low-porosity Gaussian-random-field lattice: stepping rate in
+    FP32 and, with solve, the full three-direction FP32 solve."""
+    from flowperm import solver, workloads
+    t0 = time.time()
+    m = workloads.large_grf(seed)
+    log(f"512^3 GRF: porosity {1 - m.mean():.4f} ({time.time() - t0:.1f} s to generate)")
+    from flowperm import autotune
+    TUNED["f32"] = autotune.tune(ctx, m, 3, 1, steps=10, use_cache=False)
+    log(f"autotune f32 (512^3): {({k: TUNED['f32'][k] for k in autotune.FIELDS})}")
+    log(f"  neighbour ms {TUNED['f32']['nbr_ms']}")
+    apply_tuning(ctx, "f32")
+    o = bench_steps(ctx, m, 1, 10, steps)
+    res = {"porosity": float(1 - m.mean()), "n_fluid": int((m == 0).sum()),
+           "mlups": o.n_fluid * o.steps / o.total_ms / 1e3,
+           "ms_per_step": o.total_ms / o.steps,
+           "step_kernel_GBs": o.n_fluid * 152 / (0.5 * (o.local_ms + o.nbr_ms) / 1e3) / 1e9,
+           "local_ms": o.local_ms, "neighbour_ms": o.nbr_ms,
+           "tuning": {k: TUNED["f32"][k] for k in ("local_cpt", "nbr_waves", "nbr_bpc", "nbr_mode")}}
+    log(f"512^3: {res['mlups']:.0f} MLUPS, {res['step_kernel_GBs']:.0f} GB/s")
+    if solve:
+        row = solve_logged(ctx, m, "f32", **SOLVE_CFG)
+        res["solve"] = {"seconds": row["wall_s"],
+                        "steps": [row[f"steps_{t}"] for t in ("fx", "fy", "fz")],
+                        "converged": [row[f"conv_{t}"] for t in ("fx", "fy", "fz")],
+                        "K": k_matrix(row).tolist(), "k_mean": row["k_mean"], "fa": row["fa"]}
+        log(f"512^3 solve: {res['solve']}")
+    return res
+
+
 def cpu_baseline(mask, budget_s=30.0):
     """The reference solver (LMB3d/lbm3d-perm.cpp built by oracle/Makefile)
     on this host, one single-threaded process, on a bounded sample of the same
@@ -260,12 +392,16 @@ def main():
                     help="workload mask (open = all fluid, a diagnostic)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
-    ap.add_argument("--tune", default="", help="fixed launch shapes 'cpt,waves,nt,local_bpc,nbr_bpc' "
-                    "for both precisions instead of running the autotuner")
+    ap.add_argument("--tune", default="", help="fixed launch shapes "
+                    "'cpt,waves,nt,local_bpc,nbr_bpc[,nbr_mode]' for both precisions instead of "
+                    "running the autotuner")
     ap.add_argument("--batched", type=int, default=12, help="structures in the batched-solve "
                     "sample, solved batched FP32 and plain FP64 (0 = skip)")
     ap.add_argument("--serving", type=int, default=10, help="requests in the serving sample (0 = skip)")
     ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
+    ap.add_argument("--study", default="", choices=["", "precision", "large", "large-solve"],
+                    help="instead of the default line: the 256^3 / 1024^2 precision study, or "
+                         "the 512^3 GRF lattice (stepping rate; large-solve adds the full solve)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -274,6 +410,20 @@ def main():
         sys.exit(2)
 
     from flowperm import solver
+    if a.study:
+        ctx = solver.Context()
+        from flowperm import autotune
+        mask, _ = workload_mask(256, a.seed)
+        for prec, name in ((1, "f32"), (0, "f64")):
+            TUNED[name] = autotune.tune(ctx, mask, 3, prec, steps=20, use_cache=False)
+        steps = max(2, a.steps & ~1)
+        if a.study == "precision":
+            out = {"study": "precision", **precision_study(ctx, a.seed, steps)}
+        else:
+            out = {"study": "large", **large_lattice(ctx, a.seed, steps, a.study == "large-solve")}
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return
     mask, por_target = workload_mask(a.size, a.seed, a.mask)
     n_fluid = int(mask.size - np.count_nonzero(mask))
     ctx = solver.Context()
@@ -286,9 +436,11 @@ def main():
         if prec == 0 and a.no_fp64:
             continue
         if a.tune:
-            c, w, n, lb, nb = (int(v) for v in a.tune.split(","))
+            v = [int(x) for x in a.tune.split(",")]
+            c, w, n, lb, nb = v[:5]
             tuned[name] = {"local_cpt": c, "nbr_waves": w, "nontemporal": n, "local_bpc": lb,
-                           "nbr_bpc": nb, "chunk_steps": 200, "fixed": True}
+                           "nbr_bpc": nb, "nbr_mode": v[5] if len(v) > 5 else 0,
+                           "chunk_steps": 200, "fixed": True}
             continue
         rec = autotune.tune(ctx, mask, 3, prec, steps=20, use_cache=False)
         tuned[name] = rec
@@ -298,7 +450,8 @@ def main():
     TUNED.update(tuned)
 
     def shape(r):
-        return (r["local_cpt"], r["nbr_waves"], r["nontemporal"], r["local_bpc"], r["nbr_bpc"])
+        return (r["local_cpt"], r["nbr_waves"], r["nontemporal"], r["local_bpc"], r["nbr_bpc"],
+                r.get("nbr_mode", 0))
     t32 = shape(tuned["f32"])
     log(f"FP32 D3Q19 {a.size}^3: warmup {warm}, timed {steps} steps")
     o32 = bench_steps(ctx, mask, 1, warm, steps, t32)

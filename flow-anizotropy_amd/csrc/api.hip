@@ -815,7 +815,8 @@ static bool tuning_ok(const fp_tuning *t)
     return t && (t->local_cpt == 1 || t->local_cpt == 2 || t->local_cpt == 4) &&
            t->nbr_waves >= 1 && t->nbr_waves <= 8 && t->chunk_steps >= 2 &&
            (t->nontemporal == 0 || t->nontemporal == 1) &&
-           t->local_bpc >= 0 && t->local_bpc <= 32 && t->nbr_bpc >= 0 && t->nbr_bpc <= 32;
+           t->local_bpc >= 0 && t->local_bpc <= 32 && t->nbr_bpc >= 0 && t->nbr_bpc <= 32 &&
+           (t->nbr_mode == 0 || t->nbr_mode == 1);
 }
 
 extern "C" int fp_set_tuning(fp_ctx *ctx, const fp_tuning *t)
@@ -826,6 +827,7 @@ extern "C" int fp_set_tuning(fp_ctx *ctx, const fp_tuning *t)
     ctx->tune.nontemporal = t->nontemporal;
     ctx->tune.local_bpc = t->local_bpc;
     ctx->tune.nbr_bpc = t->nbr_bpc;
+    ctx->tune.nbr_mode = t->nbr_mode;
     ctx->chunk = t->chunk_steps & ~1;
     return FP_OK;
 }
@@ -839,6 +841,7 @@ extern "C" void fp_get_tuning(fp_ctx *ctx, fp_tuning *t)
     t->nontemporal = ctx->tune.nontemporal;
     t->local_bpc = ctx->tune.local_bpc;
     t->nbr_bpc = ctx->tune.nbr_bpc;
+    t->nbr_mode = ctx->tune.nbr_mode;
 }
 
 extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
@@ -877,7 +880,7 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
     Tuning tu = ctx->tune;
     if(tune) {
         tu.local_cpt = tune->local_cpt; tu.nbr_waves = tune->nbr_waves; tu.nontemporal = tune->nontemporal;
-        tu.local_bpc = tune->local_bpc; tu.nbr_bpc = tune->nbr_bpc;
+        tu.local_bpc = tune->local_bpc; tu.nbr_bpc = tune->nbr_bpc; tu.nbr_mode = tune->nbr_mode;
     }
     b.set_tuning(tu);
     e = b.admit(&j);
@@ -933,19 +936,23 @@ extern "C" int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_p
 // ---------------------------------------------------------------------------
 // autotuner: time every launch shape of the two step kernels on one resident
 // lattice and keep the fastest of each.  Run i pairs local shape i with
-// neighbour shape i mod 12 (the kernels are timed independently by events).
+// neighbour shape i mod FP_TUNE_NNBR (the kernels are timed independently by
+// events).
 // ---------------------------------------------------------------------------
 static const int TUNE_LOCAL[24][3] = {      // cells/thread, blocks/CU, non-temporal
     {1, 2, 0}, {1, 4, 0}, {1, 8, 0}, {1, 16, 0}, {2, 2, 0}, {2, 4, 0}, {2, 8, 0}, {2, 16, 0},
     {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {4, 16, 0}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {1, 16, 1},
     {2, 2, 1}, {2, 4, 1}, {2, 8, 1}, {2, 16, 1}, {4, 2, 1}, {4, 4, 1}, {4, 8, 1}, {4, 16, 1}};
-static const int TUNE_NBR[12][2] = {        // segments/workgroup, blocks/CU (0: one per item)
-    {2, 0}, {2, 4}, {2, 8}, {2, 16}, {4, 0}, {4, 2}, {4, 4}, {4, 8}, {8, 0}, {8, 2}, {8, 3}, {8, 4}};
+// neighbour step: segments per workgroup (row-segment form), blocks/CU (0:
+// one per item), form (0 row segments, 1 compact)
+static const int TUNE_NBR[FP_TUNE_NNBR][3] = {
+    {2, 0, 0}, {2, 8, 0}, {4, 0, 0}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {8, 0, 0}, {8, 2, 0},
+    {8, 3, 0}, {8, 4, 0}, {1, 0, 1}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {1, 16, 1}, {1, 32, 1}};
 
 extern "C" void fp_tune_shape(int kind, int i, int *a, int *b, int *c)
 {
     if(kind == 0 && i >= 0 && i < 24) { *a = TUNE_LOCAL[i][0]; *b = TUNE_LOCAL[i][1]; *c = TUNE_LOCAL[i][2]; }
-    else if(kind == 1 && i >= 0 && i < 12) { *a = TUNE_NBR[i][0]; *b = TUNE_NBR[i][1]; *c = 0; }
+    else if(kind == 1 && i >= 0 && i < FP_TUNE_NNBR) { *a = TUNE_NBR[i][0]; *b = TUNE_NBR[i][1]; *c = TUNE_NBR[i][2]; }
     else { *a = *b = *c = 0; }
 }
 
@@ -978,34 +985,39 @@ extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, in
     Batch b(lattice, precision, ctx->stream, 2, false);
     e = b.admit(&j);
     if(!e.empty()) { b.detach_all(); free_job_memory(j); return fail(ctx, FP_ERR_OOM, e); }
-    double lt[24], nt[12];
+    double lt[FP_TUNE_NLOCAL], nt[FP_TUNE_NNBR];
     for(auto &v : lt) v = 1e30;
     for(auto &v : nt) v = 1e30;
-    for(int i = 0; i < 24 && e.empty(); i++) {
+    const int runs = std::max(FP_TUNE_NLOCAL, FP_TUNE_NNBR);
+    for(int r = 0; r < runs && e.empty(); r++) {
+        const int i = r % FP_TUNE_NLOCAL, n = r % FP_TUNE_NNBR;
         Tuning t = ctx->tune;
         t.local_cpt = TUNE_LOCAL[i][0]; t.local_bpc = TUNE_LOCAL[i][1]; t.nontemporal = TUNE_LOCAL[i][2];
-        t.nbr_waves = TUNE_NBR[i % 12][0]; t.nbr_bpc = TUNE_NBR[i % 12][1];
+        t.nbr_waves = TUNE_NBR[n][0]; t.nbr_bpc = TUNE_NBR[n][1]; t.nbr_mode = TUNE_NBR[n][2];
         b.set_tuning(t);
         Batch::KernelTimes kw, kt;
         e = b.time_steps(2, kw);
         if(e.empty()) e = b.time_steps(steps, kt);
         if(!e.empty()) break;
         lt[i] = std::min(lt[i], kt.n_local ? kt.local_ms / kt.n_local : 1e30);
-        nt[i % 12] = std::min(nt[i % 12], kt.n_nbr ? kt.nbr_ms / kt.n_nbr : 1e30);
+        // a form the structure cannot use (the batch fell back) is not a result
+        if(b.nbr_mode() == t.nbr_mode)
+            nt[n] = std::min(nt[n], kt.n_nbr ? kt.nbr_ms / kt.n_nbr : 1e30);
     }
     b.detach_all();
     free_job_memory(j);
     if(!e.empty()) return fail(ctx, FP_ERR_DEVICE, e);
     int bl = 0, bn = 0;
-    for(int i = 1; i < 24; i++) if(lt[i] < lt[bl]) bl = i;
-    for(int i = 1; i < 12; i++) if(nt[i] < nt[bn]) bn = i;
+    for(int i = 1; i < FP_TUNE_NLOCAL; i++) if(lt[i] < lt[bl]) bl = i;
+    for(int i = 1; i < FP_TUNE_NNBR; i++) if(nt[i] < nt[bn]) bn = i;
     best->local_cpt = TUNE_LOCAL[bl][0];
     best->local_bpc = TUNE_LOCAL[bl][1];
     best->nontemporal = TUNE_LOCAL[bl][2];
     best->nbr_waves = TUNE_NBR[bn][0];
     best->nbr_bpc = TUNE_NBR[bn][1];
+    best->nbr_mode = TUNE_NBR[bn][2];
     best->chunk_steps = ctx->chunk;
-    if(local_ms) for(int i = 0; i < 24; i++) local_ms[i] = lt[i];
-    if(nbr_ms) for(int i = 0; i < 12; i++) nbr_ms[i] = nt[i];
+    if(local_ms) for(int i = 0; i < FP_TUNE_NLOCAL; i++) local_ms[i] = lt[i];
+    if(nbr_ms) for(int i = 0; i < FP_TUNE_NNBR; i++) nbr_ms[i] = nt[i];
     return fp_set_tuning(ctx, best);
 }

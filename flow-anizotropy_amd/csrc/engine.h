@@ -26,22 +26,29 @@ struct Geometry {
     SegInfo *d_seg = nullptr;
     void *d_links = nullptr;         // LinkRec<Q-1> per segment (neighbour-step links)
     uint32_t *d_rowspec = nullptr;   // wrap-lane neighbour index per row and direction
-    size_t links_bytes = 0, rowspec_bytes = 0;
+    void *d_ctab = nullptr;          // CRec<Q-1> per fluid cell (built on first use)
+    uint32_t *d_rowbase = nullptr;   // compact-step row bases
+    size_t links_bytes = 0, rowspec_bytes = 0, ctab_bytes = 0;
     std::vector<uint8_t> solid;      // host copy, x fastest
     int refs = 0;                    // jobs using it
     ~Geometry();
     size_t device_bytes() const
     {
-        return (size_t)nunits * sizeof(SegInfo) + links_bytes + rowspec_bytes;
+        return (size_t)nunits * sizeof(SegInfo) + links_bytes + rowspec_bytes + ctab_bytes;
     }
+    // bytes the compact neighbour table of this structure takes
+    size_t compact_bytes() const;
 };
 
 // builds the segment table from a mask (x fastest; 1 = solid) and uploads it.
 // Returns an empty string or the error.
 std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz);
-// host-only check of the neighbour link tables (no device memory); *bad =
-// number of (cell, direction) pairs whose derived neighbour is wrong
+// host-only check of the neighbour link tables and of the compact neighbour
+// table (no device memory); *bad = number of (cell, direction) pairs whose
+// derived neighbour is wrong
 std::string geometry_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64_t *bad);
+// build and upload the compact neighbour table (once; no-op when present)
+std::string geometry_build_compact(Geometry &g);
 
 // One forced solve.
 struct Job {
@@ -104,6 +111,8 @@ public:
     void set_chunk(int c) { chunk_ = c < 2 ? 2 : (c & ~1); dirty_ = true; }
     void set_tuning(const Tuning &t) { tune_ = t; dirty_ = true; }
     const Tuning &tuning() const { return tune_; }
+    // neighbour-step form in use (the tuning's, unless a structure cannot use it)
+    int nbr_mode() const { return mode_; }
 
     // Timing: launch `steps` steps (even) directly, with a HIP event pair on
     // the batch stream around every kernel; returns the summed kernel times
@@ -134,6 +143,9 @@ private:
     InstState *d_state_ = nullptr;
     LocalWork *d_lwork_ = nullptr;
     NbrWork *d_nwork_ = nullptr;
+    LocalWork *d_cwork_ = nullptr;           // compact neighbour-step items (mode 1)
+    size_t cap_cwork_ = 0;
+    int mode_ = 0;                           // neighbour-step form of the current tables
     Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
     RedWork *d_lredw_ = nullptr, *d_nredw_ = nullptr;     // second-level reduction items
     RPartial *d_lred_ = nullptr, *d_nred_ = nullptr;

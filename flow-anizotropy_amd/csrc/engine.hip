@@ -23,6 +23,63 @@ Geometry::~Geometry()
     if(d_seg) (void)hipFree(d_seg);
     if(d_links) (void)hipFree(d_links);
     if(d_rowspec) (void)hipFree(d_rowspec);
+    if(d_ctab) (void)hipFree(d_ctab);
+    if(d_rowbase) (void)hipFree(d_rowbase);
+}
+
+size_t Geometry::compact_bytes() const
+{
+    const size_t rec = dim == 3 ? sizeof(CRec<18>) : sizeof(CRec<8>);
+    const int nk = dim == 3 ? 18 : 8;
+    return (size_t)n_fluid * rec + (size_t)nrows * nk * sizeof(uint32_t);
+}
+
+static int64_t round_up(int64_t v, int64_t m);
+
+// The compact neighbour table (CRec per fluid cell, row bases per row) from
+// the segment table.  False when a row holds 0xFFFF or more fluid cells (the
+// offsets are 16-bit); the row-segment neighbour step is used then.
+template<class L>
+static bool build_compact(const Geometry &g, const std::vector<SegInfo> &h,
+                          std::vector<CRec<L::Q - 1>> &tab, std::vector<uint32_t> &rowbase)
+{
+    constexpr int NK = L::Q - 1;
+    if(g.nx >= 0xFFFF) return false;
+    const int64_t stride = round_up(g.n_fluid, 1024);
+    auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+    auto rowstart = [&](int64_t r) { return h[(size_t)(r * g.nseg)].start; };
+    tab.assign((size_t)g.n_fluid, CRec<NK>{});
+    rowbase.assign((size_t)g.nrows * NK, 0u);
+    for(int64_t row = 0; row < g.nrows; row++) {
+        const int y = L::DIM == 3 ? (int)(row % g.ny) : (int)row;
+        const int z = L::DIM == 3 ? (int)(row / g.ny) : 0;
+        int64_t rr[L::Q];
+        for(int k = 1; k < L::Q; k++) {
+            const int yy = wrap(y - L::ey(k), g.ny);
+            rr[k] = L::DIM == 3 ? (int64_t)wrap(z - L::ez(k), g.nz) * g.ny + yy : yy;
+            rowbase[(size_t)row * NK + k - 1] = (uint32_t)((int64_t)L::opp(k) * stride + rowstart(rr[k]));
+        }
+        const uint8_t *own = g.solid.data() + row * g.nx;
+        uint32_t i = rowstart(row);
+        for(int x = 0; x < g.nx; x++) {
+            if(own[x]) continue;
+            CRec<NK> &R = tab[i];
+            R.row = (uint32_t)row;
+            for(int k = 1; k < L::Q; k++) {
+                const int xx = wrap(x - L::ex(k), g.nx);
+                const uint8_t *nb = g.solid.data() + rr[k] * g.nx;
+                if(nb[xx]) { R.rel[k - 1] = CREL_SOLID; continue; }
+                // fluid cells of the neighbour row before xx
+                const int sg = xx >> 6;
+                const SegInfo &s = h[(size_t)(rr[k] * g.nseg + sg)];
+                const uint64_t below = (xx & 63) ? (~0ull >> (64 - (xx & 63))) : 0ull;
+                const uint32_t idx = s.start + (uint32_t)__builtin_popcountll(s.mask & below);
+                R.rel[k - 1] = (uint16_t)(idx - rowstart(rr[k]));
+            }
+            i++;
+        }
+    }
+    return true;
 }
 
 // The neighbour links of every segment (LinkRec): which lanes' neighbours
@@ -192,13 +249,84 @@ static int64_t verify_links(const Geometry &g, const uint8_t *solid, const std::
     return bad;
 }
 
+// the compact table against a dense lookup: the slot the compact neighbour
+// step addresses for every fluid cell and direction
+template<class L>
+static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std::vector<SegInfo> &h)
+{
+    constexpr int NK = L::Q - 1;
+    std::vector<CRec<NK>> tab;
+    std::vector<uint32_t> rowbase;
+    if(!build_compact<L>(g, h, tab, rowbase)) return 0;
+    const int64_t stride = round_up(g.n_fluid, 1024);
+    std::vector<int64_t> cidx((size_t)g.nn, -1);
+    int64_t c = 0;
+    for(int64_t n = 0; n < g.nn; n++) if(solid[n] == 0) cidx[(size_t)n] = c++;
+    int64_t bad = 0;
+    for(int64_t n = 0; n < g.nn; n++) {
+        if(solid[n]) continue;
+        const int64_t i = cidx[(size_t)n];
+        const int x = (int)(n % g.nx);
+        const int64_t row = n / g.nx;
+        const int y = L::DIM == 3 ? (int)(row % g.ny) : (int)row;
+        const int z = L::DIM == 3 ? (int)(row / g.ny) : 0;
+        const CRec<NK> &R = tab[(size_t)i];
+        if((int64_t)R.row != row) { bad++; continue; }
+        for(int k = 1; k < L::Q; k++) {
+            const uint16_t rel = R.rel[k - 1];
+            const int64_t off = rel == CREL_SOLID ? (int64_t)k * stride + i
+                                                  : (int64_t)rowbase[(size_t)row * NK + k - 1] + rel;
+            const int xx = (x - L::ex(k) + g.nx) % g.nx;
+            const int yy = (y - L::ey(k) + g.ny) % g.ny;
+            const int zz = L::DIM == 3 ? (z - L::ez(k) + g.nz) % g.nz : 0;
+            const int64_t dn = ((int64_t)zz * g.ny + yy) * g.nx + xx;
+            const int64_t want = solid[(size_t)dn] ? (int64_t)k * stride + i
+                                                   : (int64_t)L::opp(k) * stride + cidx[(size_t)dn];
+            if(off != want) bad++;
+        }
+    }
+    return bad;
+}
+
 std::string geometry_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64_t *bad)
 {
     Geometry g;
     std::vector<SegInfo> h;
     std::string err = build_segments(g, solid, dim, nx, ny, nz, h);
     if(!err.empty()) return err;
-    *bad = dim == 3 ? verify_links<D3Q19>(g, solid, h) : verify_links<D2Q9>(g, solid, h);
+    g.solid.assign(solid, solid + g.nn);
+    *bad = dim == 3 ? verify_links<D3Q19>(g, solid, h) + verify_compact<D3Q19>(g, solid, h)
+                    : verify_links<D2Q9>(g, solid, h) + verify_compact<D2Q9>(g, solid, h);
+    return "";
+}
+
+std::string geometry_build_compact(Geometry &g)
+{
+    if(g.d_ctab) return "";
+    std::vector<SegInfo> h((size_t)g.nunits);
+    hipError_t e = hipMemcpy(h.data(), g.d_seg, h.size() * sizeof(SegInfo), hipMemcpyDeviceToHost);
+    if(e != hipSuccess) return hip_err("hipMemcpy(segments)", e);
+    std::vector<uint32_t> rowbase;
+    size_t tab_bytes = 0;
+    const void *tab_data = nullptr;
+    std::vector<CRec<18>> t3;
+    std::vector<CRec<8>> t2;
+    bool ok;
+    if(g.dim == 3) { ok = build_compact<D3Q19>(g, h, t3, rowbase); tab_bytes = t3.size() * sizeof(t3[0]); tab_data = t3.data(); }
+    else           { ok = build_compact<D2Q9>(g, h, t2, rowbase); tab_bytes = t2.size() * sizeof(t2[0]); tab_data = t2.data(); }
+    if(!ok) return "rows too long for the compact neighbour table";
+    e = hipMalloc(&g.d_ctab, tab_bytes);
+    if(e != hipSuccess) { g.d_ctab = nullptr; (void)hipGetLastError(); return "out of device memory"; }
+    e = hipMalloc((void **)&g.d_rowbase, rowbase.size() * sizeof(uint32_t));
+    if(e != hipSuccess) {
+        (void)hipFree(g.d_ctab); g.d_ctab = nullptr; g.d_rowbase = nullptr;
+        (void)hipGetLastError();
+        return "out of device memory";
+    }
+    e = hipMemcpy(g.d_ctab, tab_data, tab_bytes, hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(g.d_rowbase, rowbase.data(), rowbase.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if(e != hipSuccess) return hip_err("hipMemcpy(compact table)", e);
+    g.ctab_bytes = tab_bytes + rowbase.size() * sizeof(uint32_t);
     return "";
 }
 
@@ -244,6 +372,8 @@ static InstDesc make_desc(const Job &j)
     D.seg = g.d_seg;
     D.links = g.d_links;
     D.rowspec = g.d_rowspec;
+    D.ctab = g.d_ctab;
+    D.rowbase = g.d_rowbase;
     D.f = j.d_f;
     D.vel[0] = j.d_vel[0];
     D.vel[1] = j.d_vel[1];
@@ -278,6 +408,7 @@ Batch::~Batch()
     (void)hipFree(d_state_);
     (void)hipFree(d_lwork_);
     (void)hipFree(d_nwork_);
+    (void)hipFree(d_cwork_);
     (void)hipFree(d_lpart_);
     (void)hipFree(d_npart_);
     (void)hipFree(d_lredw_);
@@ -361,10 +492,17 @@ std::string Batch::rebuild()
     work_total_ = work_dead_ = 0;
     if(n_inst_ == 0) { dirty_ = false; n_lwork_ = n_nwork_ = 0; return ""; }
 
+    // the compact neighbour step needs every structure's compact table; if one
+    // cannot have it (rows of 0xFFFF+ cells, or no memory) the batch uses the
+    // row-segment step
+    mode_ = tune_.nbr_mode;
+    if(mode_ == 1)
+        for(Job *j : jobs_)
+            if(!geometry_build_compact(*j->geo).empty()) { mode_ = 0; break; }
     std::vector<InstDesc> desc(n_inst_);
     std::vector<InstCfg> cfg(n_inst_);
     std::vector<InstState> state(n_inst_);
-    std::vector<LocalWork> lw;
+    std::vector<LocalWork> lw, cw;
     std::vector<NbrWork> nw;
     for(int i = 0; i < n_inst_; i++) {
         Job *j = jobs_[i];
@@ -382,6 +520,19 @@ std::string Batch::rebuild()
             lw.push_back(w);
         }
         desc[i].lp1 = (int32_t)lw.size();
+        if(mode_ == 1) {
+            // compact neighbour step: 256 consecutive fluid cells per item
+            desc[i].np0 = (int32_t)cw.size();
+            for(int64_t c = 0; c < g.n_fluid; c += 256) {
+                LocalWork w;
+                w.inst = i;
+                w.ncell = (int32_t)std::min<int64_t>(256, g.n_fluid - c);
+                w.first = c;
+                cw.push_back(w);
+            }
+            desc[i].np1 = (int32_t)cw.size();
+            continue;
+        }
         desc[i].np0 = (int32_t)nw.size();
         {
             // slabs of nbr_waves rows stacked along z (2D: y); inside a slab
@@ -425,10 +576,10 @@ std::string Batch::rebuild()
     }
     n_lred_ = (int)lr.size();
     n_nred_ = (int)nr.size();
-    if(lw.size() > INT32_MAX || nw.size() > INT32_MAX) return "batch too large";
+    if(lw.size() > INT32_MAX || nw.size() > INT32_MAX || cw.size() > INT32_MAX) return "batch too large";
     n_lwork_ = (int)lw.size();
-    n_nwork_ = (int)nw.size();
-    work_total_ = (int64_t)lw.size() + (int64_t)nw.size();
+    n_nwork_ = (int)(mode_ == 1 ? cw.size() : nw.size());
+    work_total_ = (int64_t)lw.size() + (int64_t)n_nwork_;
     const size_t ni = (size_t)n_inst_;
     if(ni > cap_inst_ || !d_inst_) {
         (void)hipFree(d_inst_); (void)hipFree(d_cfg_); (void)hipFree(d_state_);
@@ -445,8 +596,9 @@ std::string Batch::rebuild()
     }
     if((e = ensure(d_lwork_, cap_lwork_, std::max<size_t>(1, lw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
     if((e = ensure(d_nwork_, cap_nwork_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
+    if((e = ensure(d_cwork_, cap_cwork_, std::max<size_t>(1, cw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
     if((e = ensure(d_lpart_, cap_lpart_, std::max<size_t>(1, lw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
-    if((e = ensure(d_npart_, cap_npart_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
+    if((e = ensure(d_npart_, cap_npart_, std::max<size_t>(1, (size_t)n_nwork_))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
     if((e = ensure(d_lredw_, cap_lredw_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
        (e = ensure(d_nredw_, cap_nredw_, std::max<size_t>(1, nr.size()))) != hipSuccess ||
        (e = ensure(d_lred_, cap_lred_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
@@ -457,6 +609,7 @@ std::string Batch::rebuild()
     if(e == hipSuccess) e = hipMemcpy(d_state_, state.data(), state.size() * sizeof(InstState), hipMemcpyHostToDevice);
     if(e == hipSuccess && !lw.empty()) e = hipMemcpy(d_lwork_, lw.data(), lw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !nw.empty()) e = hipMemcpy(d_nwork_, nw.data(), nw.size() * sizeof(NbrWork), hipMemcpyHostToDevice);
+    if(e == hipSuccess && !cw.empty()) e = hipMemcpy(d_cwork_, cw.data(), cw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !lr.empty()) e = hipMemcpy(d_lredw_, lr.data(), lr.size() * sizeof(RedWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !nr.empty()) e = hipMemcpy(d_nredw_, nr.data(), nr.size() * sizeof(RedWork), hipMemcpyHostToDevice);
     if(e != hipSuccess) return hip_err("hipMemcpy(tables)", e);
@@ -535,13 +688,17 @@ int64_t Batch::work_items(const Job &j) const
     const int64_t lchunk = 256 * (int64_t)tune_.local_cpt;
     const int64_t planes = g.dim == 3 ? g.nz : g.ny;
     const int64_t ustride = (g.dim == 3 ? g.ny : 1) * (int64_t)g.nseg;
-    return (g.n_fluid + lchunk - 1) / lchunk + (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
+    const int64_t nbr = mode_ == 1 ? (g.n_fluid + 255) / 256
+                                   : (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
+    return (g.n_fluid + lchunk - 1) / lchunk + nbr;
 }
 
 void Batch::launch_step(bool local)
 {
-    kern_step(lattice_, prec_, local, tune_, d_inst_, d_state_,
-              local ? (const void *)d_lwork_ : (const void *)d_nwork_,
+    Tuning tu = tune_;
+    tu.nbr_mode = mode_;
+    const void *nbr_work = mode_ == 1 ? (const void *)d_cwork_ : (const void *)d_nwork_;
+    kern_step(lattice_, prec_, local, tu, d_inst_, d_state_, local ? (const void *)d_lwork_ : nbr_work,
               local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
 }
 

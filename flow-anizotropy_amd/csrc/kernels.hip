@@ -393,6 +393,92 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 // monitor: exact flux of every instance that produced one this step, then the
 // convergence state machine.  One workgroup per instance.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// neighbour step, compact form (even t): 64 consecutive fluid cells per
+// wavefront, whatever rows they lie in.  Each cell reads its CRec (16-bit
+// offsets of its 18 neighbours inside their rows); the row bases come from
+// scalar loads when the wavefront's cells lie in at most two rows (the usual
+// case), else from vector loads.  The same slots as k_nbr are read and
+// written, so the two forms give identical results.
+// ---------------------------------------------------------------------------
+template<class L, int P>
+__global__ void __launch_bounds__(256)
+k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
+        const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part)
+{
+    using T = typename Store<L, P>::T;
+    constexpr int Q = L::Q;
+    constexpr int NK = Q - 1;
+    const ItemRange R = my_items(nwork);
+    for(int it = R.begin; it < R.end; it += R.step) {
+        const LocalWork w = work[it];
+        if(st[w.inst].phase >= PH_DONE) continue;
+        const int flux = st[w.inst].flux_next;
+        const int vslot = st[w.inst].vel_next;
+        const InstDesc &D = insts[w.inst];
+        gptr<T> f = as_global<T>(D.f);
+        const uint32_t stride = (uint32_t)D.stride;    // Q * stride < 2^32 (checked on the host)
+        const int c = (int)threadIdx.x;
+        const bool act = c < w.ncell;
+        const uint32_t i = (uint32_t)(w.first + c);
+
+        double u[3] = {0.0, 0.0, 0.0};
+        if(act) {
+            // the cell's record, as 16-byte vector loads
+            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+            constexpr int NV = (int)(sizeof(CRec<NK>) / 16);
+            union { u4 v[NV]; CRec<NK> c; } r;
+            gptr<const u4> src = as_global<const u4>(const_cast<void *>(D.ctab)) + (int64_t)i * NV;
+#pragma unroll
+            for(int q = 0; q < NV; q++) r.v[q] = src[q];
+            const uint32_t row = r.c.row;
+            // the rows of this wavefront's active cells (monotonic in the cell order)
+            const uint64_t lanes = __ballot(1);
+            const int last = 63 - __builtin_clzll(lanes);
+            const uint32_t row0 = uni(row);
+            const uint32_t row1 = __builtin_amdgcn_readlane(row, last);
+            const bool two = __ballot(row != row0 && row != row1) == 0ull;
+            uint32_t rb[NK];
+            if(two) {
+                cptr<uint32_t> b0 = as_const<uint32_t>(D.rowbase) + (int64_t)row0 * NK;
+                cptr<uint32_t> b1 = as_const<uint32_t>(D.rowbase) + (int64_t)row1 * NK;
+                const bool in0 = row == row0;
+#pragma unroll
+                for(int k = 0; k < NK; k++) {
+                    const uint32_t a0 = b0[k], a1 = b1[k];
+                    rb[k] = in0 ? a0 : a1;
+                }
+            } else {
+                gptr<const uint32_t> b = as_global<const uint32_t>((void *)const_cast<uint32_t *>(D.rowbase)) +
+                                         (int64_t)row * NK;
+#pragma unroll
+                for(int k = 0; k < NK; k++) rb[k] = b[k];
+            }
+            uint32_t off[Q];
+            off[0] = i;
+#pragma unroll
+            for(int k = 1; k < Q; k++) {
+                const uint32_t rel = r.c.rel[k - 1];
+                off[k] = rel == CREL_SOLID ? i + (uint32_t)k * stride : rb[k - 1] + rel;
+            }
+            T v[Q];
+#pragma unroll
+            for(int k = 0; k < Q; k++) v[k] = ld<false>(f + off[k]);
+            node<L, P>(v, D, u);
+#pragma unroll
+            for(int k = 0; k < Q; k++) stv<false>(f + off[L::opp(k)], v[k]);
+            if(vslot) write_vel<L>(D, vslot, (int64_t)i, u);
+        }
+        if(flux) {
+            double ch[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            int bad = 0;
+            if(act) fix_add<L::DIM>(u, ch, bad);
+            block_flux<L::DIM>(ch, bad, &part[it]);
+            __syncthreads();                           // LDS reused by the next item
+        }
+    }
+}
+
 // sum of 128-bit values across a 256-thread workgroup (result in thread 0)
 __device__ __forceinline__ void block_sum_i128(I128 (&s)[3], int &bad)
 {
@@ -555,9 +641,15 @@ static void launch_step(bool local, const Tuning &tu, const InstDesc *insts, con
     } else {
         // the neighbour kernel's scattered partial-line traffic needs the L2:
         // it never uses non-temporal accesses (measured 1.6-1.8x the bytes)
-        const NbrWork *nw = (const NbrWork *)work;
-        const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
-        hipLaunchKernelGGL((k_nbr<L, P, false>), g, b, 0, s, insts, st, nw, nwork, part);
+        if(tu.nbr_mode == 1) {
+            const LocalWork *cw = (const LocalWork *)work;
+            const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(256);
+            hipLaunchKernelGGL((k_nbr_c<L, P>), g, b, 0, s, insts, st, cw, nwork, part);
+        } else {
+            const NbrWork *nw = (const NbrWork *)work;
+            const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
+            hipLaunchKernelGGL((k_nbr<L, P, false>), g, b, 0, s, insts, st, nw, nwork, part);
+        }
     }
 }
 

@@ -175,6 +175,19 @@ template<int NK> struct LinkRec {
     uint32_t base[NK];
 };
 
+// Compact neighbour table (the alternative neighbour step that packs 64
+// fluid cells per wavefront), one record per fluid cell: for every direction
+// k >= 1, the neighbour at cell - e_k as its offset inside its own row of
+// fluid cells (0xFFFF: solid, the link bounces back), and the cell's row.
+// Per row, rowbase[row * (Q-1) + k - 1] = opp(k) * stride + the compact index
+// of the first fluid cell of that neighbour row, so the neighbour's slot is
+// rowbase + rel.  Records are padded to 16 bytes for vector loads.
+constexpr uint16_t CREL_SOLID = 0xFFFF;
+template<int NK> struct alignas(16) CRec {
+    uint16_t rel[NK];
+    uint32_t row;
+};
+
 enum Prec : int {
     PREC_F64 = 0,       // f in double, the reference's arithmetic
     PREC_F32 = 1,       // f - w in float (shifted), FP64 flux sums
@@ -184,6 +197,8 @@ enum Prec : int {
 struct InstDesc {
     const SegInfo *seg;     // [nrows * nseg]
     const void *links;      // LinkRec<Q-1>[nrows * nseg]
+    const void *ctab;       // CRec<Q-1>[n_fluid] (compact neighbour step), or null
+    const uint32_t *rowbase;   // [nrows * (Q-1)] (compact neighbour step)
     const uint32_t *rowspec;   // [nrows * (Q-1)]
     void    *f;             // populations f[k * stride + i]
     float   *vel[2];        // compact velocity fields (DIM floats per cell), or null
@@ -441,6 +456,8 @@ struct Tuning {
     int nontemporal = 0;    // local kernel: population loads/stores bypass cache allocation
     int local_bpc = 8;      // resident workgroups per CU launched (each loops over items)
     int nbr_bpc = 4;
+    int nbr_mode = 0;       // neighbour step: 0 = one wavefront per row segment (link
+                            // records), 1 = 64 fluid cells per wavefront (compact table)
 };
 
 // work of one workgroup of the local (even A-A) kernel: a run of compact cells

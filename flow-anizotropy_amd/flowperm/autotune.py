@@ -20,20 +20,27 @@ import numpy as np
 from . import _lib
 
 
-FIELDS = ("local_cpt", "nbr_waves", "chunk_steps", "nontemporal", "local_bpc", "nbr_bpc")
+FIELDS = ("local_cpt", "nbr_waves", "chunk_steps", "nontemporal", "local_bpc", "nbr_bpc",
+          "nbr_mode")
+N_LOCAL, N_NBR = 24, 16              # FP_TUNE_NLOCAL, FP_TUNE_NNBR
 
 
 def shapes(lib):
-    """The shapes fp_autotune tries: (local list of (cpt, bpc, nt), neighbour list of (waves, bpc))."""
+    """The shapes fp_autotune tries: (local list of (cpt, bpc, nt), neighbour
+    list of (waves, bpc, mode))."""
     a, b, c = C.c_int(), C.c_int(), C.c_int()
     loc, nbr = [], []
-    for i in range(24):
+    for i in range(N_LOCAL):
         lib.fp_tune_shape(0, i, C.byref(a), C.byref(b), C.byref(c))
         loc.append((a.value, b.value, c.value))
-    for i in range(12):
+    for i in range(N_NBR):
         lib.fp_tune_shape(1, i, C.byref(a), C.byref(b), C.byref(c))
-        nbr.append((a.value, b.value))
+        nbr.append((a.value, b.value, c.value))
     return loc, nbr
+
+
+def nbr_name(w, b, m):
+    return f"compact_bpc{b}" if m == 1 else f"waves{w}_bpc{b}"
 
 
 def cache_path() -> Path:
@@ -55,15 +62,16 @@ def device_name() -> str:
         return "unknown"
 
 
-def _key(lattice, precision, n_cells):
+def _key(lattice, precision, n_cells, porosity):
+    # the best neighbour-step form depends on how sparse the lattice is
     size_class = "small" if n_cells < 2_000_000 else "large"
-    return f"{device_name()}/{lattice}/{precision}/{size_class}"
+    return f"{device_name()}/{lattice}/{precision}/{size_class}/phi{round(porosity, 1):.1f}"
 
 
 def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
     """Return the best fp_tuning for this lattice class and apply it to ctx."""
     mask = np.ascontiguousarray(mask, np.uint8)
-    key = _key(lattice, precision, mask.size)
+    key = _key(lattice, precision, mask.size, float(1.0 - mask.mean()))
     cp = cache_path()
     cache = {}
     if use_cache and cp.exists():
@@ -80,8 +88,8 @@ def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
     else:
         (ny, nx), nz = mask.shape, 1
     best = _lib.Tuning()
-    lms = (C.c_double * 24)()
-    nms = (C.c_double * 12)()
+    lms = (C.c_double * N_LOCAL)()
+    nms = (C.c_double * N_NBR)()
     st = ctx.lib.fp_autotune(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, lattice, precision, steps,
                              C.byref(best), lms, nms)
     if st != 0:
@@ -89,7 +97,8 @@ def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
     rec = {k: getattr(best, k) for k in FIELDS}
     loc, nbr = shapes(ctx.lib)
     rec["local_ms"] = {f"cpt{c}_bpc{b}_nt{n}": round(v, 5) for (c, b, n), v in zip(loc, lms)}
-    rec["nbr_ms"] = {f"waves{w}_bpc{b}": round(v, 5) for (w, b), v in zip(nbr, nms)}
+    rec["nbr_ms"] = {nbr_name(w, b, m): round(v, 5) for (w, b, m), v in zip(nbr, nms)
+                     if v < 1e29}
     if use_cache:
         cache[key] = rec
         try:

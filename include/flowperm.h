@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 3
+#define FP_ABI_VERSION 4
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -256,6 +256,9 @@ typedef struct fp_tuning {
     int32_t local_bpc;        /* workgroups launched per CU, local kernel: 0..32
                                  (0: one workgroup per work item)                     */
     int32_t nbr_bpc;          /* the same for the neighbour kernel                    */
+    int32_t nbr_mode;         /* neighbour step: 0 = one wavefront per 64-cell row
+                                 segment, 1 = 64 fluid cells per wavefront (a 48-byte
+                                 per-cell neighbour table; faster on sparse lattices)  */
 } fp_tuning;
 
 /* Set / read the launch shapes a context uses for its solves and batches. */
@@ -275,11 +278,11 @@ int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, in
  * duration of every local / neighbour shape tried (FP_TUNE_NLOCAL /
  * FP_TUNE_NNBR entries, in the order of fp_tune_shape). */
 #define FP_TUNE_NLOCAL 24
-#define FP_TUNE_NNBR   12
+#define FP_TUNE_NNBR   16
 int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
                 int precision, int steps, fp_tuning *best, double *local_ms, double *nbr_ms);
 /* The i-th shape tried by fp_autotune: kind 0 = local (cpt, bpc, nontemporal in
- * a, b, c), kind 1 = neighbour (waves, bpc in a, b). */
+ * a, b, c), kind 1 = neighbour (waves, bpc, nbr_mode in a, b, c). */
 void fp_tune_shape(int kind, int i, int *a, int *b, int *c);
 
 /* ------------------------------------------------------------------------ */

@@ -148,9 +148,12 @@ def test_tune_shapes_are_valid(lib):
     for i in range(24):
         lib.fp_tune_shape(0, i, C.byref(a), C.byref(b), C.byref(c))
         assert a.value in (1, 2, 4) and 0 <= b.value <= 32 and c.value in (0, 1)
-    for i in range(12):
+    modes = set()
+    for i in range(16):
         lib.fp_tune_shape(1, i, C.byref(a), C.byref(b), C.byref(c))
-        assert 1 <= a.value <= 8 and 0 <= b.value <= 32
+        assert 1 <= a.value <= 8 and 0 <= b.value <= 32 and c.value in (0, 1)
+        modes.add(c.value)
+    assert modes == {0, 1}                      # both neighbour-step forms are tried
 
 
 def test_now_is_monotonic(lib):
@@ -159,10 +162,11 @@ def test_now_is_monotonic(lib):
 
 
 def test_neighbour_link_tables_match_a_dense_lookup(lib):
-    """the per-segment link records the neighbour kernel reads give, for every
-    fluid cell and direction, the compact index of the periodic neighbour at
-    cell - e_k (or mark it solid): random lattices of every width class
-    (x-runs of 1, < 64, 64, > 64 cells) in 3D and 2D"""
+    """the per-segment link records of the row-segment neighbour step and the
+    per-cell table of the compact one both give, for every fluid cell and
+    direction, the slot of the periodic neighbour at cell - e_k (or the cell's
+    own slot when that neighbour is solid): random lattices of every width
+    class (x-runs of 1, < 64, 64, > 64 cells) in 3D and 2D"""
     from flowperm import _lib
     rng = np.random.default_rng(0)
     shapes = [(1, 1, 1), (2, 3, 1), (3, 2, 63), (2, 2, 64), (3, 4, 65), (2, 3, 128), (5, 4, 130),
