@@ -103,13 +103,15 @@ def k_error(r32, r64):
 TUNED = {}
 
 
-def apply_tuning(ctx, name):
-    """make the autotuned launch shapes of a precision the context's"""
+def apply_tuning(ctx, name, nbr_mode=None):
+    """make the autotuned launch shapes of a precision the context's
+    (nbr_mode overrides the neighbour-step form: 2 = per structure)"""
     from flowperm import _lib
     r = TUNED.get(name)
     if r:
         t = _lib.Tuning(r["local_cpt"], r["nbr_waves"], r["chunk_steps"], r["nontemporal"],
-                        r["local_bpc"], r["nbr_bpc"], r.get("nbr_mode", 0))
+                        r["local_bpc"], r["nbr_bpc"],
+                        r.get("nbr_mode", 2) if nbr_mode is None else nbr_mode)
         ctx.lib.fp_set_tuning(ctx.handle, C.byref(t))
 
 
@@ -117,7 +119,7 @@ def plain_fp64(ctx, structs):
     """the plain path: one structure at a time, its three forcing runs one
     after another, FP64 (fp_solve3d)"""
     from flowperm import solver
-    apply_tuning(ctx, "f64")
+    apply_tuning(ctx, "f64", nbr_mode=2)       # structures of every porosity
     rows, t0 = [], time.perf_counter()
     for s in structs:
         rows.append(solver.solve_permeability3d(s, ctx=ctx, precision="f64", **SOLVE_CFG))
@@ -128,7 +130,7 @@ def burst(ctx, structs, precision="f32"):
     """every structure submitted at once to a request server: all forcing
     runs step together in one launch per step, each retiring on its own"""
     from flowperm.server import Server
-    apply_tuning(ctx, "f32" if precision == "f32" else "f64")
+    apply_tuning(ctx, "f32" if precision == "f32" else "f64", nbr_mode=2)
     with Server(3, precision, ctx=ctx) as srv:
         t0 = time.perf_counter()
         ids = [srv.submit(s, **SOLVE_CFG) for s in structs]
@@ -185,7 +187,7 @@ def serving_sample(ctx, n, seed, workers, load=0.8):
     capacity = n / tb
     rate = load * capacity
     arr = poisson_arrivals(rate, n, seed)
-    apply_tuning(ctx, "f32")
+    apply_tuning(ctx, "f32", nbr_mode=2)
     with Server(3, "f32", ctx=ctx) as srv:
         out = serve_schedule(srv, reqs, arr, SOLVE_CFG)
         st = srv.stats()

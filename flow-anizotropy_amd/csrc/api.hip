@@ -816,7 +816,7 @@ static bool tuning_ok(const fp_tuning *t)
            t->nbr_waves >= 1 && t->nbr_waves <= 8 && t->chunk_steps >= 2 &&
            (t->nontemporal == 0 || t->nontemporal == 1) &&
            t->local_bpc >= 0 && t->local_bpc <= 32 && t->nbr_bpc >= 0 && t->nbr_bpc <= 32 &&
-           (t->nbr_mode == 0 || t->nbr_mode == 1);
+           (t->nbr_mode >= 0 && t->nbr_mode <= 2);
 }
 
 extern "C" int fp_set_tuning(fp_ctx *ctx, const fp_tuning *t)

@@ -60,6 +60,7 @@ struct Job {
     void *d_f = nullptr;
     float *d_vel[2] = {nullptr, nullptr};
     bool want_fields = false;        // keep the velocity fields (3D export, 2D fieldres)
+    int nbr_mode = 0;                // neighbour-step form in use (set by the batch)
     InstState state{};
     bool admitted = false, done = false;
     int64_t id = -1;
@@ -111,7 +112,7 @@ public:
     void set_chunk(int c) { chunk_ = c < 2 ? 2 : (c & ~1); dirty_ = true; }
     void set_tuning(const Tuning &t) { tune_ = t; dirty_ = true; }
     const Tuning &tuning() const { return tune_; }
-    // neighbour-step form in use (the tuning's, unless a structure cannot use it)
+    // neighbour-step forms in use: 0 row segments, 1 compact, 2 both
     int nbr_mode() const { return mode_; }
 
     // Timing: launch `steps` steps (even) directly, with a HIP event pair on
@@ -145,7 +146,8 @@ private:
     NbrWork *d_nwork_ = nullptr;
     LocalWork *d_cwork_ = nullptr;           // compact neighbour-step items (mode 1)
     size_t cap_cwork_ = 0;
-    int mode_ = 0;                           // neighbour-step form of the current tables
+    int mode_ = 0;                           // forms in the tables: 0, 1 or 2 (both)
+    int n_cwork_ = 0;
     Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
     RedWork *d_lredw_ = nullptr, *d_nredw_ = nullptr;     // second-level reduction items
     RPartial *d_lred_ = nullptr, *d_nred_ = nullptr;

@@ -492,13 +492,20 @@ std::string Batch::rebuild()
     work_total_ = work_dead_ = 0;
     if(n_inst_ == 0) { dirty_ = false; n_lwork_ = n_nwork_ = 0; return ""; }
 
-    // the compact neighbour step needs every structure's compact table; if one
-    // cannot have it (rows of 0xFFFF+ cells, or no memory) the batch uses the
-    // row-segment step
-    mode_ = tune_.nbr_mode;
-    if(mode_ == 1)
-        for(Job *j : jobs_)
-            if(!geometry_build_compact(*j->geo).empty()) { mode_ = 0; break; }
+    // neighbour-step form of every instance: the tuning's, or by porosity
+    // (auto); the compact form needs the structure's compact table, and an
+    // instance whose structure cannot have it (rows of 0xFFFF+ cells, or no
+    // memory) uses the row-segment form
+    int n_mode[2] = {0, 0};
+    for(Job *j : jobs_) {
+        const Geometry &g = *j->geo;
+        int m = tune_.nbr_mode;
+        if(m == 2) m = (double)g.n_fluid < COMPACT_BELOW_POROSITY * (double)g.nn ? 1 : 0;
+        if(m == 1 && !geometry_build_compact(*j->geo).empty()) m = 0;
+        j->nbr_mode = m;
+        n_mode[m]++;
+    }
+    mode_ = n_mode[1] == 0 ? 0 : (n_mode[0] == 0 ? 1 : 2);
     std::vector<InstDesc> desc(n_inst_);
     std::vector<InstCfg> cfg(n_inst_);
     std::vector<InstState> state(n_inst_);
@@ -520,8 +527,9 @@ std::string Batch::rebuild()
             lw.push_back(w);
         }
         desc[i].lp1 = (int32_t)lw.size();
-        if(mode_ == 1) {
-            // compact neighbour step: 256 consecutive fluid cells per item
+        if(j->nbr_mode == 1) {
+            // compact neighbour step: 256 consecutive fluid cells per item; its
+            // partial slots follow the row-segment ones (offset fixed below)
             desc[i].np0 = (int32_t)cw.size();
             for(int64_t c = 0; c < g.n_fluid; c += 256) {
                 LocalWork w;
@@ -556,6 +564,11 @@ std::string Batch::rebuild()
         }
         desc[i].np1 = (int32_t)nw.size();
     }
+    for(int i = 0; i < n_inst_; i++)
+        if(jobs_[i]->nbr_mode == 1) {
+            desc[i].np0 += (int32_t)nw.size();
+            desc[i].np1 += (int32_t)nw.size();
+        }
     // second-level reduction items over each instance's partial ranges
     std::vector<RedWork> lr, nr;
     for(int i = 0; i < n_inst_; i++) {
@@ -578,8 +591,9 @@ std::string Batch::rebuild()
     n_nred_ = (int)nr.size();
     if(lw.size() > INT32_MAX || nw.size() > INT32_MAX || cw.size() > INT32_MAX) return "batch too large";
     n_lwork_ = (int)lw.size();
-    n_nwork_ = (int)(mode_ == 1 ? cw.size() : nw.size());
-    work_total_ = (int64_t)lw.size() + (int64_t)n_nwork_;
+    n_nwork_ = (int)nw.size();
+    n_cwork_ = (int)cw.size();
+    work_total_ = (int64_t)lw.size() + (int64_t)n_nwork_ + (int64_t)n_cwork_;
     const size_t ni = (size_t)n_inst_;
     if(ni > cap_inst_ || !d_inst_) {
         (void)hipFree(d_inst_); (void)hipFree(d_cfg_); (void)hipFree(d_state_);
@@ -598,7 +612,7 @@ std::string Batch::rebuild()
     if((e = ensure(d_nwork_, cap_nwork_, std::max<size_t>(1, nw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
     if((e = ensure(d_cwork_, cap_cwork_, std::max<size_t>(1, cw.size()))) != hipSuccess) return hip_err("hipMalloc(work)", e);
     if((e = ensure(d_lpart_, cap_lpart_, std::max<size_t>(1, lw.size()))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
-    if((e = ensure(d_npart_, cap_npart_, std::max<size_t>(1, (size_t)n_nwork_))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
+    if((e = ensure(d_npart_, cap_npart_, std::max<size_t>(1, (size_t)n_nwork_ + n_cwork_))) != hipSuccess) return hip_err("hipMalloc(partials)", e);
     if((e = ensure(d_lredw_, cap_lredw_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
        (e = ensure(d_nredw_, cap_nredw_, std::max<size_t>(1, nr.size()))) != hipSuccess ||
        (e = ensure(d_lred_, cap_lred_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
@@ -688,18 +702,28 @@ int64_t Batch::work_items(const Job &j) const
     const int64_t lchunk = 256 * (int64_t)tune_.local_cpt;
     const int64_t planes = g.dim == 3 ? g.nz : g.ny;
     const int64_t ustride = (g.dim == 3 ? g.ny : 1) * (int64_t)g.nseg;
-    const int64_t nbr = mode_ == 1 ? (g.n_fluid + 255) / 256
-                                   : (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
+    const int64_t nbr = j.nbr_mode == 1 ? (g.n_fluid + 255) / 256
+                                        : (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
     return (g.n_fluid + lchunk - 1) / lchunk + nbr;
 }
 
 void Batch::launch_step(bool local)
 {
+    if(local) {
+        kern_step(lattice_, prec_, true, tune_, d_inst_, d_state_, d_lwork_, n_lwork_, d_lpart_, stream_);
+        return;
+    }
+    // the row-segment items, then the compact ones (their partial slots follow)
     Tuning tu = tune_;
-    tu.nbr_mode = mode_;
-    const void *nbr_work = mode_ == 1 ? (const void *)d_cwork_ : (const void *)d_nwork_;
-    kern_step(lattice_, prec_, local, tu, d_inst_, d_state_, local ? (const void *)d_lwork_ : nbr_work,
-              local ? n_lwork_ : n_nwork_, local ? d_lpart_ : d_npart_, stream_);
+    if(n_nwork_ > 0) {
+        tu.nbr_mode = 0;
+        kern_step(lattice_, prec_, false, tu, d_inst_, d_state_, d_nwork_, n_nwork_, d_npart_, stream_);
+    }
+    if(n_cwork_ > 0) {
+        tu.nbr_mode = 1;
+        kern_step(lattice_, prec_, false, tu, d_inst_, d_state_, d_cwork_, n_cwork_,
+                  d_npart_ + n_nwork_, stream_);
+    }
 }
 
 std::string Batch::time_steps(int steps, KernelTimes &kt)

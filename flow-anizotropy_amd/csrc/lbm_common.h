@@ -456,9 +456,16 @@ struct Tuning {
     int nontemporal = 0;    // local kernel: population loads/stores bypass cache allocation
     int local_bpc = 8;      // resident workgroups per CU launched (each loops over items)
     int nbr_bpc = 4;
-    int nbr_mode = 0;       // neighbour step: 0 = one wavefront per row segment (link
-                            // records), 1 = 64 fluid cells per wavefront (compact table)
+    int nbr_mode = 2;       // neighbour step: 0 = one wavefront per row segment (link
+                            // records), 1 = 64 fluid cells per wavefront (compact table),
+                            // 2 = per instance by porosity (COMPACT_BELOW_POROSITY)
 };
+
+// Below this porosity the compact neighbour step is the faster form: per fluid
+// cell its cost is flat, the row-segment step's grows as 1/porosity (measured
+// on MI355X, FP32 D3Q19: 48.6 ps/cell compact at porosity 0.62 and 0.14;
+// row segments 43.3 ps at 0.62, 82.6 ps at 0.14; equal near 0.42).
+constexpr double COMPACT_BELOW_POROSITY = 0.42;
 
 // work of one workgroup of the local (even A-A) kernel: a run of compact cells
 struct LocalWork {

@@ -99,3 +99,32 @@ def test_compact_batch_mix_equals_alone(compact_ctx):
     for i, rid in enumerate(ids):
         for k in KEYS:
             assert res[rid][0][k] == alone[i][k], k
+
+
+def test_auto_form_mixed_batch_equals_row_segment(gpu_ctx):
+    """nbr_mode 2 (the default): a dense and a sparse structure in one batch
+    take different forms (both kernels run each even step) and every result
+    equals the row-segment form's bit for bit"""
+    import ctypes as C
+    from flowperm import _lib, masks, solver
+    from flowperm.server import Server
+    dense = masks.spheres((20, 20, 24), 0.7, 3, 5, seed=5)
+    sparse = masks.grf((24, 28, 70), porosity=0.3, corr=(3, 3, 6), seed=6)
+    print("porosities", 1 - dense.mean(), 1 - sparse.mean(), flush=True)
+    assert 1 - sparse.mean() < 0.42 < 1 - dense.mean()
+    cfg = dict(ff=40, eps=1e-6, lag=200, min_steps=400, avg_steps=200, max_steps=2000)
+    seg = solver.Context()
+    t = _lib.Tuning()
+    seg.lib.fp_get_tuning(seg.handle, C.byref(t))
+    t.nbr_mode = 0
+    assert seg.lib.fp_set_tuning(seg.handle, C.byref(t)) == 0
+    try:
+        ref = [solver.solve_permeability3d(m, ctx=seg, precision="f32", **cfg) for m in (dense, sparse)]
+    finally:
+        seg.close()
+    with Server(3, "f32", ctx=gpu_ctx) as s:
+        ids = [s.submit(m, **cfg) for m in (dense, sparse)]
+        res = s.run(ids)
+    for i, rid in enumerate(ids):
+        for k in KEYS:
+            assert res[rid][0][k] == ref[i][k], (i, k)
