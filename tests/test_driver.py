@@ -1,0 +1,141 @@
+"""
+The dataset driver (flowperm.driver), the in-process mirror of
+scripts/solve_permeability3d.py: restartable per-sample files, verify /
+repair, shard and interleave order, the merge's derived targets, manifest and
+label-noise ceilings.  The bookkeeping runs on the CPU with rows the reference
+binary wrote (tests/golden/); the solve itself is a GPU test.
+"""
+import hashlib
+import json
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from conftest import GOLDEN, REPO
+
+NAMES = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5", "sph_x28y24z20_s1", "c1_64_s0"]
+
+
+def make_dataset(root, names=NAMES, with_rows=True):
+    (root / "structures").mkdir(parents=True)
+    (root / "perm").mkdir()
+    rows = []
+    for sid, name in enumerate(names):
+        shutil.copy(GOLDEN / f"{name}.raw", root / "structures" / f"{name}.raw")
+        rows.append({"sample_id": sid, "filename": f"{name}.raw", "stratum": sid % 2,
+                     "rock": "a" if sid < 3 else "b"})
+        if with_rows:
+            shutil.copy(GOLDEN / f"{name}.csv", root / "perm" / f"{sid:06d}.csv")
+    pd.DataFrame(rows).to_csv(root / "structures.csv", index=False)
+    return root
+
+
+def test_valid_result_rejects_incomplete_files(tmp_path):
+    from flowperm.driver import valid_result
+    good = GOLDEN / "sph16_s0.csv"
+    assert valid_result(good)
+    text = good.read_text().splitlines(keepends=True)
+    (tmp_path / "hdr.csv").write_text(text[0])                       # killed after the header
+    (tmp_path / "empty.csv").write_text("")
+    nan = text[1].split(",")
+    nan[text[0].split(",").index("K_xx")] = "nan"
+    (tmp_path / "nan.csv").write_text(text[0] + ",".join(nan))
+    for f in ("hdr.csv", "empty.csv", "nan.csv"):
+        assert not valid_result(tmp_path / f), f
+    assert not valid_result(tmp_path / "absent.csv")
+
+
+def test_verify_and_repair(tmp_path):
+    from flowperm.driver import per_sample_path, verify
+    root = make_dataset(tmp_path / "ds")
+    perm = root / "perm"
+    (perm / "000001.csv").write_text((GOLDEN / "sph16_s0.csv").read_text().splitlines()[0] + "\n")
+    (perm / "000002.csv").unlink()
+    (perm / "000003.csv.tmp.4242").write_text("partial")
+    expected = [(i, per_sample_path(str(perm), i)) for i in range(len(NAMES))]
+    lines = []
+    r = verify(str(perm), expected, ("K_xx", "K_yy", "K_zz", "conv_fx"), log=lines.append)
+    assert r["complete"] == [0, 3, 4] and r["incomplete"] == [1] and r["missing"] == [2]
+    assert len(r["strays"]) == 1
+    r = verify(str(perm), expected, ("K_xx", "K_yy", "K_zz", "conv_fx"), repair=True,
+               shard_ids={3}, log=lines.append)
+    assert not (perm / "000001.csv").exists() and not (perm / "000003.csv.tmp.4242").exists()
+    # a shard never removes another shard's temporary files
+    (perm / "000004.csv.tmp.1").write_text("x")
+    r = verify(str(perm), expected, ("K_xx",), repair=True, shard_ids={0, 1}, log=lines.append)
+    assert (perm / "000004.csv.tmp.1").exists()
+
+
+def test_interleave_order():
+    from flowperm.driver import interleaved
+    df = pd.DataFrame({"sample_id": range(7), "rock": list("aaabbbc")})
+    assert list(interleaved(df, "rock").sample_id) == [0, 3, 6, 1, 4, 2, 5]
+
+
+def test_merge_derived_targets_manifest_and_ceilings(tmp_path, capsys):
+    from flowperm import driver
+    root = make_dataset(tmp_path / "ds")
+    df = pd.read_csv(root / "structures.csv")
+    out = driver.merge(str(root), df, str(root / "perm"), 3)
+    ref = pd.concat([pd.read_csv(GOLDEN / f"{n}.csv") for n in NAMES], ignore_index=True)
+    assert list(out.sample_id) == list(range(len(NAMES)))
+    trace = ref.K_xx + ref.K_yy + ref.K_zz
+    assert np.allclose(out.trace, trace)
+    assert np.allclose(out.koff_rel_err_xy, (ref.K_xy - ref.K_yx).abs() / (2 * ref.k_xy).abs(),
+                       equal_nan=True)
+    assert np.allclose(out.dev_xx, ref.K_xx / ref.k_mean - 1)
+    assert np.allclose(out.dev_yz, ref.k_yz / ref.k_mean)
+    assert np.allclose(out.log_k_ratio, np.log(ref.k1 / ref.k3))
+    assert list(out.conv_all) == [int(a and b and c) for a, b, c in
+                                  zip(ref.conv_fx, ref.conv_fy, ref.conv_fz)]
+    merged = pd.read_csv(root / "permeability.csv")
+    assert len(merged) == len(NAMES) and "rock" in merged.columns
+    man = json.loads((root / "permeability_manifest.json").read_text())
+    assert man["n_rows"] == len(NAMES) and man["contiguous"]
+    assert man["sha256_ids"] == hashlib.sha256(b"0,1,2,3,4").hexdigest()
+    ceil = driver.koff_ceilings3d(out)
+    err = (out.K_xz - out.K_zx).abs() / 2
+    assert ceil["k_xz"] == pytest.approx(1 - (err ** 2).mean() / out.k_xz.var())
+    assert "R^2 <=" in capsys.readouterr().out
+
+
+def test_driver_command_line_verify_and_merge(tmp_path):
+    root = make_dataset(tmp_path / "ds")
+    p = subprocess.run([sys.executable, "-m", "flowperm.driver", "--dataset", str(root),
+                        "--verify"], capture_output=True, text=True,
+                       cwd=REPO / "flow-anizotropy_amd", timeout=300)
+    assert p.returncode == 0 and "complete   5" in p.stdout
+    p = subprocess.run([sys.executable, "-m", "flowperm.driver", "--dataset", str(root),
+                        "--merge-only"], capture_output=True, text=True,
+                       cwd=REPO / "flow-anizotropy_amd", timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert (root / "permeability.csv").exists()
+    p = subprocess.run([sys.executable, "-m", "flowperm.driver", "--dataset", str(root),
+                        "--merge-only", "--shard", "0/2"], capture_output=True, text=True,
+                       cwd=REPO / "flow-anizotropy_amd", timeout=300)
+    assert p.returncode != 0 and "exclude each other" in p.stderr
+
+
+@pytest.mark.gpu
+def test_driver_solves_missing_samples_like_the_executable(tmp_path):
+    """a resumed run solves only what is missing, through the batched server,
+    and each per-sample file is the drop-in executable's text"""
+    from flowperm import driver
+    root = make_dataset(tmp_path / "ds", names=NAMES[:3], with_rows=False)
+    flags = "--lag 200 --eps 1e-6 --min-steps 400 --avg-steps 200 --max-steps 5000"
+    shutil.copy(GOLDEN / "sph16_s0.csv", root / "perm" / "000000.csv")      # already done
+    assert driver.main(["--dataset", str(root), "--ff", "40", "--extra", flags]) == 0
+    assert (root / "perm" / "000000.csv").read_text() == (GOLDEN / "sph16_s0.csv").read_text()
+    for sid, name in list(enumerate(NAMES[:3]))[1:]:
+        out = tmp_path / f"exe{sid}.csv"
+        structure = str(root / "structures" / f"{name}.raw")
+        rc = subprocess.run([sys.executable, str(REPO / "flow-anizotropy_amd" / "bin" / "lbm3d-perm"),
+                             structure, str(out), "--quiet", "--ff", "40", *flags.split()],
+                            timeout=600).returncode
+        assert rc in (0, 3)
+        assert (root / "perm" / f"{sid:06d}.csv").read_text() == out.read_text()
+    assert len(pd.read_csv(root / "permeability.csv")) == 3
