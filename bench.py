@@ -126,18 +126,53 @@ def plain_fp64(ctx, structs):
     return rows, time.perf_counter() - t0
 
 
-def burst(ctx, structs, precision="f32"):
+def burst(ctx, structs, precision="f32", every=20.0):
     """every structure submitted at once to a request server: all forcing
-    runs step together in one launch per step, each retiring on its own"""
+    runs step together in one launch per step, each retiring on its own
+    (a progress line every `every` seconds)"""
     from flowperm.server import Server
     apply_tuning(ctx, "f32" if precision == "f32" else "f64", nbr_mode=2)
     with Server(3, precision, ctx=ctx) as srv:
         t0 = time.perf_counter()
         ids = [srv.submit(s, **SOLVE_CFG) for s in structs]
-        res = srv.run(ids)
+        last = time.perf_counter()
+        while srv.step():
+            if time.perf_counter() - last > every:
+                last = time.perf_counter()
+                st = srv.stats()
+                log(f"  ... {st['completed']}/{len(ids)} done, {st['live_instances']} runs live, "
+                    f"{st['steps']} steps, {last - t0:.0f} s")
+        res = {i: srv.poll(i) for i in ids}
         dt = time.perf_counter() - t0
         st = srv.stats()
     return [res[i][0] for i in ids], dt, st
+
+
+def batched_full(ctx, n, m, seed, workers):
+    """the whole batched set: n structures x 3 forcing runs resident and
+    stepped together in FP32, against the plain FP64 path on the first m"""
+    from flowperm import workloads
+    t0 = time.time()
+    structs = workloads.generate("batched", n, seed, workers)
+    fluid = np.array([int((s == 0).sum()) for s in structs])
+    log(f"batched-full: {n} structures ({fluid.sum() / 1e6:.0f} M fluid cells in all) "
+        f"generated in {time.time() - t0:.0f} s")
+    rows32, t32, st = burst(ctx, structs)
+    log(f"batched FP32: {n} tensors in {t32:.1f} s = {n / t32:.3f} tensors/s, "
+        f"peak device memory {st['bytes_peak'] / 2**30:.1f} GiB")
+    rows64, t64 = plain_fp64(ctx, structs[:m])
+    err = max(k_error(a, b) for a, b in zip(rows32[:m], rows64))
+    t32m = sum(r["fluid_updates"] for r in rows32[:m]) / (st["fluid_updates"] / t32)
+    return {"structures": n, "instances": 3 * n, "fluid_cells_total": int(fluid.sum()),
+            "batched_fp32_tensors_per_s": n / t32, "batched_fp32_seconds": t32,
+            "batched_fp32_mlups": st["fluid_updates"] / t32 / 1e6,
+            "peak_device_bytes": st["bytes_peak"], "chunks": st["chunks"],
+            "plain_fp64_structures": m, "plain_fp64_tensors_per_s": m / t64,
+            "plain_fp64_mlups": sum(r["fluid_updates"] for r in rows64) / t64 / 1e6,
+            "speedup_same_first_m": t64 / t32m,
+            "speedup_note": "plain FP64 time on the first m against the batched FP32 time "
+                            "those m structures' updates take at the batch's aggregate rate",
+            "max_k_error_first_m": err}
 
 
 def batched_sample(ctx, n, seed, workers):
@@ -401,7 +436,8 @@ def main():
                     "sample, solved batched FP32 and plain FP64 (0 = skip)")
     ap.add_argument("--serving", type=int, default=10, help="requests in the serving sample (0 = skip)")
     ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
-    ap.add_argument("--study", default="", choices=["", "precision", "large", "large-solve"],
+    ap.add_argument("--study", default="", choices=["", "precision", "large", "large-solve",
+                                                     "batched-full"],
                     help="instead of the default line: the 256^3 / 1024^2 precision study, or "
                          "the 512^3 GRF lattice (stepping rate; large-solve adds the full solve)")
     a = ap.parse_args()
@@ -421,6 +457,8 @@ def main():
         steps = max(2, a.steps & ~1)
         if a.study == "precision":
             out = {"study": "precision", **precision_study(ctx, a.seed, steps)}
+        elif a.study == "batched-full":
+            out = {"study": "batched-full", **batched_full(ctx, 512, 24, a.seed, a.workers)}
         else:
             out = {"study": "large", **large_lattice(ctx, a.seed, steps, a.study == "large-solve")}
         ctx.close()

@@ -452,10 +452,10 @@ std::string Batch::admit(Job *j)
             }
         }
     }
-    j->geo->refs++;
     kern_rest(lattice_, prec_, make_desc(*j), stream_);
     e = hipGetLastError();
-    if(e != hipSuccess) return hip_err("rest-state kernel", e);
+    if(e != hipSuccess) { free_job(j); return hip_err("rest-state kernel", e); }
+    j->geo->refs++;
     init_state(j->cfg, j->state);
     j->admitted = true;
     j->done = j->state.phase >= PH_DONE;

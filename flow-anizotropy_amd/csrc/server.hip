@@ -54,11 +54,15 @@ struct fp_server {
 
 namespace {
 
-// device memory a request holds while resident: its segment table and the
+// device memory a request holds while resident: its segment tables, the
+// compact neighbour table when its structure will use that form, and the
 // population (and, in 2D, velocity snapshot) arrays of each of its runs
-size_t request_bytes(const Geometry &g, int lattice, int prec, int njobs)
+size_t request_bytes(const Geometry &g, int lattice, int prec, int njobs, int nbr_mode)
 {
-    return g.device_bytes() + (size_t)njobs * job_bytes(g, lattice, prec, lattice == 2);
+    const bool compact = nbr_mode == 1 ||
+                         (nbr_mode == 2 && (double)g.n_fluid < COMPACT_BELOW_POROSITY * (double)g.nn);
+    return g.device_bytes() + (compact ? g.compact_bytes() : 0) +
+           (size_t)njobs * job_bytes(g, lattice, prec, lattice == 2);
 }
 
 int64_t submit(fp_server *S, std::unique_ptr<Request> r, const uint8_t *solid)
@@ -72,7 +76,7 @@ int64_t submit(fp_server *S, std::unique_ptr<Request> r, const uint8_t *solid)
         return -(int64_t)status_of_geometry_error(e);
     }
     const int njobs = r->nruns + (r->validate ? 1 : 0);
-    r->bytes = request_bytes(*r->geo, S->lattice, S->prec, njobs);
+    r->bytes = request_bytes(*r->geo, S->lattice, S->prec, njobs, S->batch->tuning().nbr_mode);
     if((int64_t)r->bytes > S->budget) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->st.rejected++;
