@@ -443,8 +443,9 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this solver has no multi-GPU path (batches shard by "
-              "instance with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this solver has no multi-GPU path (a batch would "
+              "shard by instance with no exchange between GPUs). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     from flowperm import solver

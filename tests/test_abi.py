@@ -7,7 +7,6 @@ points that do no device work behave as the reference does.
 import ctypes as C
 import re
 import subprocess
-from pathlib import Path
 
 import numpy as np
 import pytest
