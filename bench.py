@@ -122,7 +122,8 @@ def plain_fp64(ctx, structs):
     apply_tuning(ctx, "f64", nbr_mode=2)       # structures of every porosity
     rows, t0 = [], time.perf_counter()
     for s in structs:
-        rows.append(solver.solve_permeability3d(s, ctx=ctx, precision="f64", **SOLVE_CFG))
+        rows.append(solver.solve_permeability3d(s, ctx=ctx, precision="f64", concurrent=False,
+                                                **SOLVE_CFG))
     return rows, time.perf_counter() - t0
 
 
@@ -454,7 +455,7 @@ def main():
         from flowperm import autotune
         mask, _ = workload_mask(256, a.seed)
         for prec, name in ((1, "f32"), (0, "f64")):
-            TUNED[name] = autotune.tune(ctx, mask, 3, prec, steps=20, use_cache=False)
+            TUNED[name] = autotune.tune(ctx, mask, 3, prec, steps=40, use_cache=False)
         steps = max(2, a.steps & ~1)
         if a.study == "precision":
             out = {"study": "precision", **precision_study(ctx, a.seed, steps)}
@@ -483,7 +484,7 @@ def main():
                            "nbr_bpc": nb, "nbr_mode": v[5] if len(v) > 5 else 0,
                            "chunk_steps": 200, "fixed": True}
             continue
-        rec = autotune.tune(ctx, mask, 3, prec, steps=20, use_cache=False)
+        rec = autotune.tune(ctx, mask, 3, prec, steps=40, use_cache=False)
         tuned[name] = rec
         log(f"autotune {name}: best {({k: rec[k] for k in autotune.FIELDS})}")
         log(f"  local ms {rec['local_ms']}")

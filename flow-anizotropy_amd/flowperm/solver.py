@@ -134,7 +134,7 @@ def _precision(p):
 
 
 def config3d(ff=1.0, eps=1e-5, lag=2000, hold=2, min_steps=4000, max_steps=200000,
-             avg_steps=2000, validate=False, precision="f64", concurrent=False):
+             avg_steps=2000, validate=False, precision="f64", concurrent=True):
     c = _lib.Config3D()
     c.ff, c.eps, c.lag, c.hold = float(ff), float(eps), int(lag), int(hold)
     c.min_steps, c.max_steps, c.avg_steps = int(min_steps), int(max_steps), int(avg_steps)
@@ -183,7 +183,9 @@ def solve_permeability3d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
     """One lbm3d-perm invocation on a mask (array [z,y,x] or .raw path).
 
     Keyword arguments: ff, eps, lag, hold, min_steps, max_steps, avg_steps
-    (the reference flags), validate, precision ("f64" | "f32"), concurrent.
+    (the reference flags), validate, precision ("f64" | "f32"), concurrent (default
+    True: the three forcing runs step together; the results are bitwise those of
+    running them one after another, as the reference does).
     Returns the CSV row as a dict plus 'exit_code', 'status' and, with
     validate, 'val_*' entries; with velocity=True also 'velocity'
     (3, nz, ny, nx, 3) float32.
@@ -251,7 +253,7 @@ def row3d(r, st, ff, filename):
 
 def config2d(ff=1.0, eps=1e-3, lag=5000, hold=3, min_steps=10000, max_steps=500000,
              avg_steps=20000, avg_sample=50, qfloor=1e-2, ksigma=3.0, L0=None,
-             validate=None, precision="f32ref", concurrent=False, shape=None):
+             validate=None, precision="f32ref", concurrent=True, shape=None):
     c = _lib.Config2D()
     c.ff, c.eps, c.qfloor, c.ksigma = float(ff), float(eps), float(qfloor), float(ksigma)
     if L0 is None:                     # lbm.h:24-27: L0 = 4 at 256, scaled with the lattice
