@@ -14,10 +14,10 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
 void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDesc *insts,
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
 void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s);
-void kern_reduce(int nwork, const RedWork *work, const InstState *st, const Partial *part,
-                 RPartial *out, hipStream_t s);
-void kern_monitor(int ninst, const InstDesc *insts, const InstCfg *cfgs, InstState *st,
-                  const RPartial *lred, const RPartial *nred, hipStream_t s);
+// after a step: exact flux sums and the convergence update of every instance
+void kern_finish(int nwork, const RedWork *work, const InstDesc *insts, const InstCfg *cfgs,
+                 InstState *st, const Partial *part, RPartial *red, unsigned int *counter,
+                 bool local, hipStream_t s);
 
 // One voxel structure resident on the device: the segment table.
 struct Geometry {
@@ -127,6 +127,7 @@ public:
 private:
     std::string rebuild();
     void launch_step(bool local);
+    void launch_finish(bool local);
     bool wants_rebuild() const;
     int64_t work_items(const Job &j) const;
     void release_graph();
@@ -152,6 +153,8 @@ private:
     RedWork *d_lredw_ = nullptr, *d_nredw_ = nullptr;     // second-level reduction items
     RPartial *d_lred_ = nullptr, *d_nred_ = nullptr;
     int n_lred_ = 0, n_nred_ = 0;
+    unsigned int *d_count_ = nullptr;        // per-instance finished-item counters
+    size_t cap_count_ = 0;
     size_t cap_lredw_ = 0, cap_nredw_ = 0, cap_lred_ = 0, cap_nred_ = 0;
     InstState *h_state_ = nullptr;       // pinned mirror
     int n_inst_ = 0, n_lwork_ = 0, n_nwork_ = 0;

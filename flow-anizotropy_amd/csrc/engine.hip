@@ -415,6 +415,7 @@ Batch::~Batch()
     (void)hipFree(d_nredw_);
     (void)hipFree(d_lred_);
     (void)hipFree(d_nred_);
+    (void)hipFree(d_count_);
     if(h_state_) (void)hipHostFree(h_state_);
 }
 
@@ -616,7 +617,8 @@ std::string Batch::rebuild()
     if((e = ensure(d_lredw_, cap_lredw_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
        (e = ensure(d_nredw_, cap_nredw_, std::max<size_t>(1, nr.size()))) != hipSuccess ||
        (e = ensure(d_lred_, cap_lred_, std::max<size_t>(1, lr.size()))) != hipSuccess ||
-       (e = ensure(d_nred_, cap_nred_, std::max<size_t>(1, nr.size()))) != hipSuccess)
+       (e = ensure(d_nred_, cap_nred_, std::max<size_t>(1, nr.size()))) != hipSuccess ||
+       (e = ensure(d_count_, cap_count_, (size_t)n_inst_)) != hipSuccess)
         return hip_err("hipMalloc(reduction)", e);
     e = hipMemcpy(d_inst_, desc.data(), desc.size() * sizeof(InstDesc), hipMemcpyHostToDevice);
     if(e == hipSuccess) e = hipMemcpy(d_cfg_, cfg.data(), cfg.size() * sizeof(InstCfg), hipMemcpyHostToDevice);
@@ -626,6 +628,11 @@ std::string Batch::rebuild()
     if(e == hipSuccess && !cw.empty()) e = hipMemcpy(d_cwork_, cw.data(), cw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !lr.empty()) e = hipMemcpy(d_lredw_, lr.data(), lr.size() * sizeof(RedWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !nr.empty()) e = hipMemcpy(d_nredw_, nr.data(), nr.size() * sizeof(RedWork), hipMemcpyHostToDevice);
+    if(e == hipSuccess) {
+        // the finish kernel's counters start at zero (and it leaves them so)
+        const std::vector<unsigned int> zero((size_t)n_inst_, 0u);
+        e = hipMemcpy(d_count_, zero.data(), zero.size() * sizeof(unsigned int), hipMemcpyHostToDevice);
+    }
     if(e != hipSuccess) return hip_err("hipMemcpy(tables)", e);
     dirty_ = false;
     return "";
@@ -652,9 +659,7 @@ std::string Batch::run_chunk()
         for(int t = 0; t < chunk_; t++) {
             const bool local = (t & 1) == 0;     // every live job starts a chunk at an even step count
             launch_step(local);
-            kern_reduce(local ? n_lred_ : n_nred_, local ? d_lredw_ : d_nredw_, d_state_,
-                        local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, stream_);
-            kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lred_, d_nred_, stream_);
+            launch_finish(local);
         }
     };
     if(use_graph_) {
@@ -707,6 +712,12 @@ int64_t Batch::work_items(const Job &j) const
     return (g.n_fluid + lchunk - 1) / lchunk + nbr;
 }
 
+void Batch::launch_finish(bool local)
+{
+    kern_finish(local ? n_lred_ : n_nred_, local ? d_lredw_ : d_nredw_, d_inst_, d_cfg_, d_state_,
+                local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, d_count_, local, stream_);
+}
+
 void Batch::launch_step(bool local)
 {
     if(local) {
@@ -745,9 +756,7 @@ std::string Batch::time_steps(int steps, KernelTimes &kt)
         (void)hipEventRecord(ev[k++], stream_);
         launch_step(local);
         (void)hipEventRecord(ev[k++], stream_);
-        kern_reduce(local ? n_lred_ : n_nred_, local ? d_lredw_ : d_nredw_, d_state_,
-                    local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, stream_);
-        kern_monitor(n_inst_, d_inst_, d_cfg_, d_state_, d_lred_, d_nred_, stream_);
+        launch_finish(local);
         (void)hipEventRecord(ev[k++], stream_);
         (void)hipEventRecord(ev[k++], stream_);
     }

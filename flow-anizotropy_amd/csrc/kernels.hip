@@ -500,84 +500,84 @@ __device__ __forceinline__ void block_sum_i128(I128 (&s)[3], int &bad)
 }
 
 // ---------------------------------------------------------------------------
-// first reduction level of the exact flux: every workgroup folds up to
-// REDUCE_SPAN step-kernel partials of one instance into 128-bit sums, so the
-// per-instance monitor adds a few hundred values, not a lattice's worth.
+// After every step: the exact flux of every instance that produced one, then
+// its convergence state machine, in one launch.  Each workgroup takes one
+// reduction item: up to REDUCE_SPAN step-kernel partials of one instance,
+// folded into 128-bit sums (on flux steps).  Every workgroup then counts
+// itself in on a per-instance counter; the last one of an instance (so, after
+// every workgroup of that instance has read the state) adds the instance's
+// 128-bit sums, rounds q once, advances the state and resets the counter.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_reduce(const RedWork *__restrict__ work, const InstState *__restrict__ st,
-         const Partial *__restrict__ part, RPartial *__restrict__ out)
+__device__ void monitor_update(const InstCfg &c, InstState *st, const double q[3], int flux)
 {
-    const RedWork w = work[blockIdx.x];
-    if(st[w.inst].phase >= PH_DONE || !st[w.inst].flux_next) return;
-    I128 s[3] = {i128_zero(), i128_zero(), i128_zero()};
-    int bad = 0;
-    for(int p = w.p0 + (int)threadIdx.x; p < w.p1; p += blockDim.x) {
-        const Partial pp = part[p];
-        for(int k = 0; k < 3; k++)
-            s[k] = i128_add(s[k], i128_from_chunks(pp.c[k][0], pp.c[k][1], pp.c[k][2]));
-        bad |= pp.bad;
-    }
-    block_sum_i128(s, bad);
-    if(threadIdx.x == 0) {
-        RPartial r;
-        for(int k = 0; k < 3; k++) r.s[k] = s[k];
-        r.bad = bad;
-        r.pad[0] = r.pad[1] = r.pad[2] = 0;
-        out[blockIdx.x] = r;
-    }
+    InstState S = *st;
+    advance_state(c, S, q, flux);
+    plan_next(c, S);
+    *st = S;
 }
 
-// ---------------------------------------------------------------------------
-// monitor: exact flux of every instance that produced one this step, then the
-// convergence state machine.  One workgroup per instance.
-// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_monitor(const InstDesc *__restrict__ insts, const InstCfg *__restrict__ cfgs,
-          InstState *__restrict__ st, const RPartial *__restrict__ lred,
-          const RPartial *__restrict__ nred)
+k_finish(const RedWork *__restrict__ work, const InstDesc *__restrict__ insts,
+         const InstCfg *__restrict__ cfgs, InstState *__restrict__ st,
+         const Partial *__restrict__ part, RPartial *__restrict__ red,
+         unsigned int *__restrict__ counter, int local)
 {
-    const int inst = blockIdx.x;
-    // thread 0 rewrites st[inst] at the end: read it once, through LDS
-    __shared__ int sh_phase, sh_flux;
-    __shared__ int64_t sh_t;
+    const RedWork w = work[blockIdx.x];
+    __shared__ int sh_phase, sh_flux, sh_last;
     if(threadIdx.x == 0) {
-        sh_phase = st[inst].phase;
-        sh_flux = st[inst].flux_next;
-        sh_t = st[inst].s + 1;
+        sh_phase = st[w.inst].phase;
+        sh_flux = st[w.inst].flux_next;
     }
     __syncthreads();
-    const int phase = sh_phase;
-    if(phase >= PH_DONE) return;
+    if(sh_phase >= PH_DONE) return;                // the same for every item of the instance
     const int flux = sh_flux;
-    const int64_t t = sh_t;
-    const InstDesc &D = insts[inst];
-
+    const InstDesc &D = insts[w.inst];
+    const int r0 = local ? D.rl0 : D.rn0, r1 = local ? D.rl1 : D.rn1;
+    I128 s[3] = {i128_zero(), i128_zero(), i128_zero()};
+    int bad = 0;
+    if(flux) {
+        for(int p = w.p0 + (int)threadIdx.x; p < w.p1; p += blockDim.x) {
+            const Partial pp = part[p];
+            for(int k = 0; k < 3; k++)
+                s[k] = i128_add(s[k], i128_from_chunks(pp.c[k][0], pp.c[k][1], pp.c[k][2]));
+            bad |= pp.bad;
+        }
+        block_sum_i128(s, bad);
+    }
+    if(threadIdx.x == 0) {
+        if(flux) {
+            RPartial r;
+            for(int k = 0; k < 3; k++) r.s[k] = s[k];
+            r.bad = bad;
+            r.pad[0] = r.pad[1] = r.pad[2] = 0;
+            red[blockIdx.x] = r;
+        }
+        __threadfence();                           // the sums are visible before the count
+        const unsigned int n = atomicAdd(&counter[w.inst], 1u);
+        sh_last = n == (unsigned int)(r1 - r0 - 1);
+    }
+    __syncthreads();
+    if(!sh_last) return;
+    __threadfence();                               // every item's sums are visible now
     double q[3] = {0.0, 0.0, 0.0};
     if(flux) {
-        const bool local = (t & 1) != 0;
-        const RPartial *P = local ? lred : nred;
-        const int p0 = local ? D.rl0 : D.rn0, p1 = local ? D.rl1 : D.rn1;
-        I128 s[3] = {i128_zero(), i128_zero(), i128_zero()};
-        int bad = 0;
-        for(int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
-            const RPartial &pp = P[p];
+        s[0] = s[1] = s[2] = i128_zero();
+        bad = 0;
+        for(int p = r0 + (int)threadIdx.x; p < r1; p += blockDim.x) {
+            const RPartial pp = red[p];
             for(int k = 0; k < 3; k++) s[k] = i128_add(s[k], pp.s[k]);
             bad |= pp.bad;
         }
         block_sum_i128(s, bad);
-        if(threadIdx.x == 0) {
-            const InstCfg &c = cfgs[inst];
+    }
+    if(threadIdx.x == 0) {
+        const InstCfg &c = cfgs[w.inst];
+        if(flux) {
             for(int k = 0; k < 3; k++) q[k] = i128_to_double_scaled(s[k]) / c.denom[k];
             if(bad) q[0] = q[1] = q[2] = __builtin_nan("");
         }
-    }
-    if(threadIdx.x == 0) {
-        const InstCfg &c = cfgs[inst];
-        InstState S = st[inst];
-        advance_state(c, S, q, flux);
-        plan_next(c, S);
-        st[inst] = S;
+        counter[w.inst] = 0u;                      // ready for the next step
+        monitor_update(c, &st[w.inst], q, flux);
     }
 }
 
@@ -705,18 +705,13 @@ void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDe
     }
 }
 
-void kern_reduce(int nwork, const RedWork *work, const InstState *st, const Partial *part,
-                 RPartial *out, hipStream_t s)
+void kern_finish(int nwork, const RedWork *work, const InstDesc *insts, const InstCfg *cfgs,
+                 InstState *st, const Partial *part, RPartial *red, unsigned int *counter,
+                 bool local, hipStream_t s)
 {
     if(nwork <= 0) return;
-    hipLaunchKernelGGL(k_reduce, dim3(nwork), dim3(256), 0, s, work, st, part, out);
-}
-
-void kern_monitor(int ninst, const InstDesc *insts, const InstCfg *cfgs, InstState *st,
-                  const RPartial *lred, const RPartial *nred, hipStream_t s)
-{
-    if(ninst <= 0) return;
-    hipLaunchKernelGGL(k_monitor, dim3(ninst), dim3(256), 0, s, insts, cfgs, st, lred, nred);
+    hipLaunchKernelGGL(k_finish, dim3(nwork), dim3(256), 0, s, work, insts, cfgs, st, part, red,
+                       counter, local ? 1 : 0);
 }
 
 } // namespace fp
