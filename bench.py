@@ -377,6 +377,90 @@ def large_lattice(ctx, seed, steps, solve):
     return res
 
 
+def split_errors(r32, r64, dirs, err, suffix):
+    """max K error of the FP32 rows against FP64, apart for structures whose
+    FP64 runs all converged and for the rest: a structure with no through
+    path never converges and its K is a transient near zero, so an error
+    relative to it says little about the arithmetic"""
+    conv = [all(b[f"conv_{d}"] for d in dirs) for b in r64]
+    e = [err(a, b) for a, b in zip(r32, r64)]
+    ok = [x for x, c in zip(e, conv) if c]
+    bad = [x for x, c in zip(e, conv) if not c]
+    return {f"max_k_error_f32_vs_f64_converged{suffix}": max(ok) if ok else None,
+            f"max_k_error_f32_vs_f64_unconverged{suffix}": max(bad) if bad else None,
+            f"unconverged_f64{suffix}": len(bad)}
+
+
+def small_batched(ctx, seed, n2=48, n3=24):
+    """The reference's own dataset sizes: D2Q9 at 256^2 (its fixed lattice,
+    2D defaults) and D3Q19 at 80^3 (--ff 40 --lag 1000 --eps 1e-6), each
+    batched in one request server against the plain one-at-a-time path (the
+    runs of a structure one after another)."""
+    from flowperm import masks, solver
+    res = {}
+    rng = np.random.default_rng(seed)
+    m2 = [masks.ellipses((256, 256), float(rng.uniform(0.4, 0.8)), seed=int(rng.integers(1 << 30)))
+          for _ in range(n2)]
+    m3 = [masks.ellipsoids((80, 80, 80), float(rng.uniform(0.4, 0.8)), seed=int(rng.integers(1 << 30)))
+          for _ in range(n3)]
+    from flowperm.server import Server
+
+    def batched(ms, lattice, precision, cfg):
+        apply_tuning(ctx, "f32", nbr_mode=2)
+        with Server(lattice, precision, ctx=ctx) as srv:
+            t0 = time.perf_counter()
+            ids = [srv.submit(m, **cfg) for m in ms]
+            last = t0
+            while srv.step():
+                if time.perf_counter() - last > 20:
+                    last = time.perf_counter()
+                    log(f"  ... batched {lattice}D {precision}: {srv.stats()['completed']}/{len(ms)} done")
+            rows = [srv.poll(i)[0] for i in ids]
+            return rows, time.perf_counter() - t0
+
+    def plain(ms, lattice, precision, cfg, limit):
+        apply_tuning(ctx, "f64" if precision == "f64" else "f32", nbr_mode=2)
+        t0 = time.perf_counter()
+        rows = []
+        for m in ms[:limit]:
+            f = solver.solve_permeability3d if lattice == 3 else solver.solve_permeability2d
+            rows.append(f(m, ctx=ctx, precision=precision, concurrent=False, **cfg))
+        return rows, time.perf_counter() - t0
+
+    cfg2 = {}
+    r32, t32 = batched(m2, 2, "f32", cfg2)
+    rref, tref = batched(m2, 2, "f32ref", cfg2)
+    p64, t64 = plain(m2, 2, "f64", cfg2, 8)
+    pref, tpref = plain(m2, 2, "f32ref", cfg2, 8)
+    res["d2q9_256"] = {
+        "structures": n2, "config": "ellipses 3:1, porosity U(0.4, 0.8), reference 2D defaults",
+        "batched_f32_tensors_per_s": n2 / t32, "batched_f32ref_tensors_per_s": n2 / tref,
+        "plain_f64_tensors_per_s": 8 / t64, "plain_f32ref_tensors_per_s": 8 / tpref,
+        "speedup_f32_vs_plain_f64": (n2 / t32) / (8 / t64),
+        "speedup_f32ref_batched_vs_plain": (n2 / tref) / (8 / tpref),
+        **split_errors(r32, p64, ("fx", "fy"), k2_error, ""),
+        "f32ref_rows_equal_plain": all(a["k_xx"] == b["k_xx"] and a["k_yy"] == b["k_yy"]
+                                       for a, b in zip(rref, pref)),
+        "steps_differ_f32_vs_f64": sum((a["steps_fx"], a["steps_fy"]) != (b["steps_fx"], b["steps_fy"])
+                                       for a, b in zip(r32, p64))}
+    # the default eps 1e-3 lets the FP32 and FP64 runs stop at different
+    # steps; at a tight eps the difference left is the arithmetic's
+    tight = dict(eps=1e-6)
+    t32r, _ = batched(m2[:8], 2, "f32", tight)
+    t64r, _ = batched(m2[:8], 2, "f64", tight)
+    res["d2q9_256"].update(split_errors(t32r, t64r, ("fx", "fy"), k2_error, "_eps1e-6"))
+    log(f"2D 256^2: {res['d2q9_256']}")
+    r32, t32 = batched(m3, 3, "f32", SOLVE_CFG)
+    p64, t64 = plain(m3, 3, "f64", SOLVE_CFG, 6)
+    res["d3q19_80"] = {
+        "structures": n3, "config": "ellipsoids 3:1, porosity U(0.4, 0.8), --ff 40 --lag 1000 --eps 1e-6",
+        "batched_f32_tensors_per_s": n3 / t32, "plain_f64_tensors_per_s": 6 / t64,
+        "speedup_f32_vs_plain_f64": (n3 / t32) / (6 / t64),
+        **split_errors(r32, p64, ("fx", "fy", "fz"), k_error, "")}
+    log(f"3D 80^3: {res['d3q19_80']}")
+    return res
+
+
 def cpu_baseline(mask, budget_s=30.0):
     """The reference solver (LMB3d/lbm3d-perm.cpp built by oracle/Makefile)
     on this host, one single-threaded process, on a bounded sample of the same
@@ -438,7 +522,7 @@ def main():
     ap.add_argument("--serving", type=int, default=10, help="requests in the serving sample (0 = skip)")
     ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
     ap.add_argument("--study", default="", choices=["", "precision", "large", "large-solve",
-                                                     "batched-full"],
+                                                     "batched-full", "small-batched"],
                     help="instead of the default line: the 256^3 / 1024^2 precision study, or "
                          "the 512^3 GRF lattice (stepping rate; large-solve adds the full solve)")
     a = ap.parse_args()
@@ -459,6 +543,8 @@ def main():
         steps = max(2, a.steps & ~1)
         if a.study == "precision":
             out = {"study": "precision", **precision_study(ctx, a.seed, steps)}
+        elif a.study == "small-batched":
+            out = {"study": "small-batched", **small_batched(ctx, a.seed)}
         elif a.study == "batched-full":
             out = {"study": "batched-full", **batched_full(ctx, 512, 24, a.seed, a.workers)}
         else:
