@@ -143,6 +143,13 @@ def test_velocity_export_matches_oracle_and_reference(gpu_ctx, oracle):
         scale = np.abs(ref).max()
         assert np.abs(vel[d] - ref).max() <= 1e-6 * scale, t
         assert np.all(vel[d][mask == 1] == 0)           # mask-zero
+        # flux reproduction (scripts/solve_velocity3d.py:196-210): the mean of the
+        # exported float32 field over the whole cell is the row's q to ~1e-7, since
+        # the FP64 run sits at a fixed point
+        qd = abs(r[f"q{'xyz'[d]}_{t}"])
+        for c, comp in enumerate("xyz"):
+            q = float(vel[d][..., c].astype(np.float64).mean())
+            assert abs(q - r[f"q{comp}_{t}"]) <= 1e-6 * qd, (t, comp, q, r[f"q{comp}_{t}"])
 
 
 def test_mask_errors_map_to_reference_exit_codes(gpu_ctx):
