@@ -35,6 +35,7 @@ size_t Geometry::compact_bytes() const
 }
 
 static int64_t round_up(int64_t v, int64_t m);
+static int64_t pop_stride(int64_t n_fluid);
 
 // The compact neighbour table (CRec per fluid cell, row bases per row) from
 // the segment table.  False when a row holds 0xFFFF or more fluid cells (the
@@ -45,7 +46,7 @@ static bool build_compact(const Geometry &g, const std::vector<SegInfo> &h,
 {
     constexpr int NK = L::Q - 1;
     if(g.nx >= 0xFFFF) return false;
-    const int64_t stride = round_up(g.n_fluid, 1024);
+    const int64_t stride = pop_stride(g.n_fluid);
     auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
     auto rowstart = [&](int64_t r) { return h[(size_t)(r * g.nseg)].start; };
     tab.assign((size_t)g.n_fluid, CRec<NK>{});
@@ -258,7 +259,7 @@ static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std
     std::vector<CRec<NK>> tab;
     std::vector<uint32_t> rowbase;
     if(!build_compact<L>(g, h, tab, rowbase)) return 0;
-    const int64_t stride = round_up(g.n_fluid, 1024);
+    const int64_t stride = pop_stride(g.n_fluid);
     std::vector<int64_t> cidx((size_t)g.nn, -1);
     int64_t c = 0;
     for(int64_t n = 0; n < g.nn; n++) if(solid[n] == 0) cidx[(size_t)n] = c++;
