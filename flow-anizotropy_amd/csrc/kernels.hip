@@ -390,10 +390,6 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 }
 
 // ---------------------------------------------------------------------------
-// monitor: exact flux of every instance that produced one this step, then the
-// convergence state machine.  One workgroup per instance.
-// ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
 // neighbour step, compact form (even t): 64 consecutive fluid cells per
 // wavefront, whatever rows they lie in.  Each cell reads its CRec (16-bit
 // offsets of its 18 neighbours inside their rows); the row bases come from

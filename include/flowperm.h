@@ -242,7 +242,8 @@ typedef struct fp_bench_out {
     double  total_ms;         /* event time of the whole timed region       */
     double  local_ms;         /* average duration: local (odd) step kernel  */
     double  nbr_ms;           /* average duration: neighbour (even) kernel  */
-    double  monitor_ms;       /* average duration: convergence monitor      */
+    double  monitor_ms;       /* average duration: flux reduction and       */
+                              /* convergence update after a step (k_finish) */
     double  bytes_per_update; /* algorithmic bytes per fluid update: 2 Q s  */
 } fp_bench_out;
 
