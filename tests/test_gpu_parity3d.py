@@ -143,13 +143,19 @@ def test_velocity_export_matches_oracle_and_reference(gpu_ctx, oracle):
         scale = np.abs(ref).max()
         assert np.abs(vel[d] - ref).max() <= 1e-6 * scale, t
         assert np.all(vel[d][mask == 1] == 0)           # mask-zero
-        # flux reproduction (scripts/solve_velocity3d.py:196-210): the mean of the
-        # exported float32 field over the whole cell is the row's q to ~1e-7, since
-        # the FP64 run sits at a fixed point
+        # flux reproduction (scripts/solve_velocity3d.py:196-210): the exported field
+        # is the last step's, q the mean over the averaging window.  This run keeps
+        # an odd/even oscillation of its flux (se_q * sqrt(avg_steps) = 2.5e-11, 5e-5
+        # of q), so the field's mean reproduces q to that spread, not to the ~1e-7 a
+        # steady run gives; and it leaves the reference's own field the same residual
         qd = abs(r[f"q{'xyz'[d]}_{t}"])
+        n_avg = cfg.get("avg_steps", 2000)
         for c, comp in enumerate("xyz"):
             q = float(vel[d][..., c].astype(np.float64).mean())
-            assert abs(q - r[f"q{comp}_{t}"]) <= 1e-6 * qd, (t, comp, q, r[f"q{comp}_{t}"])
+            spread = r[f"se_q{comp}_{t}"] * np.sqrt(n_avg)
+            assert abs(q - r[f"q{comp}_{t}"]) <= 2 * spread + 1e-7 * qd, (t, comp)
+            ref_resid = float(ref[..., c].astype(np.float64).mean()) - row[f"q{comp}_{t}"]
+            assert abs((q - r[f"q{comp}_{t}"]) - ref_resid) <= 1e-6 * qd, (t, comp)
 
 
 def test_mask_errors_map_to_reference_exit_codes(gpu_ctx):
