@@ -139,3 +139,38 @@ def test_driver_solves_missing_samples_like_the_executable(tmp_path):
         assert rc in (0, 3)
         assert (root / "perm" / f"{sid:06d}.csv").read_text() == out.read_text()
     assert len(pd.read_csv(root / "permeability.csv")) == 3
+
+
+@pytest.mark.gpu
+def test_driver_2d_solves_like_the_executable(tmp_path):
+    """--dim 2 (scripts/solve_permeability.py): GIF structures of a non-square
+    lattice solved through the server; each per-sample file is lbm2d-perm's
+    text and the merge carries the 2D derived targets (:119-131)"""
+    from flowperm import driver, masks, structio
+    root = tmp_path / "ds2"
+    (root / "structures").mkdir(parents=True)
+    (root / "perm").mkdir()
+    rows = []
+    for sid in range(3):
+        m = masks.ellipses((48, 64), 0.6, seed=sid)                  # [y, x]
+        name = f"s{sid}.gif"
+        structio.save_gif2d(root / "structures" / name, m)
+        rows.append({"sample_id": sid, "filename": name, "stratum": sid % 2})
+    pd.DataFrame(rows).to_csv(root / "structures.csv", index=False)
+    flags = "--lag 1000 --min-steps 2000 --avg-steps 1000 --max-steps 20000"
+    assert driver.main(["--dataset", str(root), "--dim", "2", "--ff", "1",
+                        "--extra", flags]) == 0
+    for sid in range(3):
+        out = tmp_path / f"exe{sid}.csv"
+        structure = str(root / "structures" / f"s{sid}.gif")
+        rc = subprocess.run([sys.executable, str(REPO / "flow-anizotropy_amd" / "bin" / "lbm2d-perm"),
+                             structure, str(out), "--quiet", "--ff", "1", *flags.split()],
+                            timeout=600).returncode
+        assert rc in (0, 3)
+        assert (root / "perm" / f"{sid:06d}.csv").read_text() == out.read_text()
+    merged = pd.read_csv(root / "permeability.csv")
+    assert list(merged.sample_id) == [0, 1, 2] and "stratum" in merged.columns
+    assert np.allclose(merged.cos2theta ** 2 + merged.sin2theta ** 2, 1.0)
+    assert np.allclose(merged.k_ratio, merged.k1 / merged.k2)
+    assert (merged.k_ratio >= 1.0).all()
+    assert np.allclose(merged.koff_over_trace, merged.k_off.abs() / (merged.k_xx + merged.k_yy).abs())
