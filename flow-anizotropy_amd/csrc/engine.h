@@ -123,6 +123,10 @@ public:
         int64_t n_local = 0, n_nbr = 0, n_monitor = 0;
     };
     std::string time_steps(int steps, KernelTimes &kt);
+    // Timing as solves run: `steps` steps (even) as replays of one captured
+    // chunk graph (plus one for the remainder), after one untimed replay;
+    // `ms` is the event time of the timed replays.
+    std::string graph_steps(int steps, float &ms);
 
 private:
     std::string rebuild();

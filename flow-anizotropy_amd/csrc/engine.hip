@@ -785,6 +785,62 @@ std::string Batch::time_steps(int steps, KernelTimes &kt)
     return "";
 }
 
+std::string Batch::graph_steps(int steps, float &ms)
+{
+    ms = 0.0f;
+    if(wants_rebuild()) {
+        std::string err = rebuild();
+        if(!err.empty()) return err;
+    }
+    if(n_inst_ == 0) return "nothing to step";
+    steps &= ~1;
+    if(steps < 2) return "";
+    const int c = std::min(steps, 200) & ~1;         // every replay starts at an even step
+    const int reps = steps / c, rest = steps - reps * c;
+    hipGraph_t g[2] = {nullptr, nullptr};
+    hipGraphExec_t x[2] = {nullptr, nullptr};
+    hipEvent_t a = nullptr, b = nullptr;
+    auto capture = [&](int n, hipGraph_t &gg, hipGraphExec_t &xx) -> std::string {
+        hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+        if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
+        for(int t = 0; t < n; t++) {
+            const bool local = (t & 1) == 0;
+            launch_step(local);
+            launch_finish(local);
+        }
+        e = hipStreamEndCapture(stream_, &gg);
+        if(e != hipSuccess) { gg = nullptr; return hip_err("hipStreamEndCapture", e); }
+        e = hipGraphInstantiate(&xx, gg, nullptr, nullptr, 0);
+        if(e != hipSuccess) { xx = nullptr; return hip_err("hipGraphInstantiate", e); }
+        return "";
+    };
+    std::string err = capture(c, g[0], x[0]);
+    if(err.empty() && rest) err = capture(rest, g[1], x[1]);
+    if(err.empty() && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess))
+        err = "hipEventCreate failed";
+    if(err.empty()) {
+        hipError_t e = hipGraphLaunch(x[0], stream_);                // untimed
+        if(e == hipSuccess) e = hipEventRecord(a, stream_);
+        for(int r = 0; r < reps && e == hipSuccess; r++) e = hipGraphLaunch(x[0], stream_);
+        if(e == hipSuccess && rest) e = hipGraphLaunch(x[1], stream_);
+        if(e == hipSuccess) e = hipEventRecord(b, stream_);
+        if(e == hipSuccess) e = hipStreamSynchronize(stream_);
+        if(e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if(e != hipSuccess) err = hip_err("graph replay", e);
+    }
+    if(a) (void)hipEventDestroy(a);
+    if(b) (void)hipEventDestroy(b);
+    for(int i = 0; i < 2; i++) {
+        if(x[i]) (void)hipGraphExecDestroy(x[i]);
+        if(g[i]) (void)hipGraphDestroy(g[i]);
+    }
+    if(!err.empty()) return err;
+    hipError_t e = hipMemcpy(h_state_, d_state_, n_inst_ * sizeof(InstState), hipMemcpyDeviceToHost);
+    if(e != hipSuccess) return hip_err("hipMemcpy(state)", e);
+    for(int i = 0; i < n_inst_; i++) if(jobs_[i]) jobs_[i]->state = h_state_[i];
+    return "";
+}
+
 std::vector<Job *> Batch::take_done()
 {
     std::vector<Job *> out;

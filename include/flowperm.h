@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 4
+#define FP_ABI_VERSION 5
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -239,12 +239,16 @@ typedef struct fp_bench_out {
     int64_t n_fluid;          /* fluid cells (one update each per step)     */
     int64_t n_cells;          /* all lattice cells                          */
     int64_t steps;            /* timed steps                                */
-    double  total_ms;         /* event time of the whole timed region       */
+    double  total_ms;         /* event time of the timed steps, replayed as */
+                              /* hipGraph chunks as solves run them         */
     double  local_ms;         /* average duration: local (odd) step kernel  */
     double  nbr_ms;           /* average duration: neighbour (even) kernel  */
     double  monitor_ms;       /* average duration: flux reduction and       */
                               /* convergence update after a step (k_finish) */
     double  bytes_per_update; /* algorithmic bytes per fluid update: 2 Q s  */
+    double  eager_total_ms;   /* the same steps launched one by one with an */
+                              /* event pair around every kernel (the pass   */
+                              /* the per-kernel averages come from)         */
 } fp_bench_out;
 
 /* Launch shapes of the step kernels; the autotuner picks them per device.  A new
@@ -269,9 +273,12 @@ typedef struct fp_tuning {
 int  fp_set_tuning(fp_ctx *ctx, const fp_tuning *t);
 void fp_get_tuning(fp_ctx *ctx, fp_tuning *t);
 
-/* tune: NULL = the context's shapes.  flux_every_step: 0 = convergence checks
- * at the default lag (2000 steps), as in the first phase of a solve; 1 = the
- * exact flux sum on every step, as in the averaging phase. */
+/* Steps one resident instance: `warmup` then `steps` steps launched one by one
+ * with events around every kernel (the per-kernel averages), then `steps` more
+ * as hipGraph chunk replays (total_ms).  tune: NULL = the context's shapes.
+ * flux_every_step: 0 = convergence checks at the default lag (2000 steps), as
+ * in the first phase of a solve; 1 = the exact flux sum on every step, as in
+ * the averaging phase. */
 int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
                    int precision, const double F[3], int warmup, int steps,
                    const fp_tuning *tune, int flux_every_step, fp_bench_out *out);
