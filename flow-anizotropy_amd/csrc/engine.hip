@@ -35,7 +35,6 @@ size_t Geometry::compact_bytes() const
 }
 
 static int64_t round_up(int64_t v, int64_t m);
-static int64_t pop_stride(int64_t n_fluid);
 
 // The compact neighbour table (CRec per fluid cell, row bases per row) from
 // the segment table.  False when a row holds 0xFFFF or more fluid cells (the
@@ -46,7 +45,6 @@ static bool build_compact(const Geometry &g, const std::vector<SegInfo> &h,
 {
     constexpr int NK = L::Q - 1;
     if(g.nx >= 0xFFFF) return false;
-    const int64_t stride = pop_stride(g.n_fluid);
     auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
     auto rowstart = [&](int64_t r) { return h[(size_t)(r * g.nseg)].start; };
     tab.assign((size_t)g.n_fluid, CRec<NK>{});
@@ -58,7 +56,7 @@ static bool build_compact(const Geometry &g, const std::vector<SegInfo> &h,
         for(int k = 1; k < L::Q; k++) {
             const int yy = wrap(y - L::ey(k), g.ny);
             rr[k] = L::DIM == 3 ? (int64_t)wrap(z - L::ez(k), g.nz) * g.ny + yy : yy;
-            rowbase[(size_t)row * NK + k - 1] = (uint32_t)((int64_t)L::opp(k) * stride + rowstart(rr[k]));
+            rowbase[(size_t)row * NK + k - 1] = (uint32_t)rowstart(rr[k]);
         }
         const uint8_t *own = g.solid.data() + row * g.nx;
         uint32_t i = rowstart(row);
@@ -259,7 +257,6 @@ static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std
     std::vector<CRec<NK>> tab;
     std::vector<uint32_t> rowbase;
     if(!build_compact<L>(g, h, tab, rowbase)) return 0;
-    const int64_t stride = pop_stride(g.n_fluid);
     std::vector<int64_t> cidx((size_t)g.nn, -1);
     int64_t c = 0;
     for(int64_t n = 0; n < g.nn; n++) if(solid[n] == 0) cidx[(size_t)n] = c++;
@@ -275,14 +272,16 @@ static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std
         if((int64_t)R.row != row) { bad++; continue; }
         for(int k = 1; k < L::Q; k++) {
             const uint16_t rel = R.rel[k - 1];
-            const int64_t off = rel == CREL_SOLID ? (int64_t)k * stride + i
-                                                  : (int64_t)rowbase[(size_t)row * NK + k - 1] + rel;
+            const uint32_t off = rel == CREL_SOLID
+                                     ? pop_at<L::Q>(k, (uint32_t)i)
+                                     : pop_at<L::Q>(L::opp(k), rowbase[(size_t)row * NK + k - 1] + rel);
             const int xx = (x - L::ex(k) + g.nx) % g.nx;
             const int yy = (y - L::ey(k) + g.ny) % g.ny;
             const int zz = L::DIM == 3 ? (z - L::ez(k) + g.nz) % g.nz : 0;
             const int64_t dn = ((int64_t)zz * g.ny + yy) * g.nx + xx;
-            const int64_t want = solid[(size_t)dn] ? (int64_t)k * stride + i
-                                                   : (int64_t)L::opp(k) * stride + cidx[(size_t)dn];
+            const uint32_t want = solid[(size_t)dn]
+                                      ? pop_at<L::Q>(k, (uint32_t)i)
+                                      : pop_at<L::Q>(L::opp(k), (uint32_t)cidx[(size_t)dn]);
             if(off != want) bad++;
         }
     }

@@ -2,7 +2,8 @@
 // storage, exact flux partials, and the device-side convergence monitor.
 //
 // Storage.  Fluid cells are numbered in the .raw order (z, y, x fastest)
-// skipping solid ones; population k of cell i lives at f[k*stride + i].
+// skipping solid ones; population k of cell i lives at f[pop_at<Q>(k, i)]
+// (blocks of 128 cells, each holding its Q populations one after another).
 // Solid cells are never stored.  Each lattice row is cut into 64-cell
 // segments; SegInfo holds the segment's fluid bits and the compact index of
 // its first fluid cell, from which any neighbour's index and solidity follow
@@ -182,7 +183,7 @@ k_rest(InstDesc D)
         else if constexpr(L::DIM == 3) w = (T)L::w(k);
         else if constexpr(P == PREC_F64) w = (T)(k == 0 ? 4.0/9.0 : (k < 5 ? 1.0/9.0 : 1.0/36.0));
         else w = (T)L::wf(k);
-        f[k * D.stride + i] = w;
+        f[pop_at<L::Q>(k, (uint32_t)i)] = w;
     }
 }
 
@@ -204,8 +205,9 @@ __device__ __forceinline__ ItemRange my_items(int nitems)
 // ---------------------------------------------------------------------------
 // local step (odd t): each thread updates CPT consecutive compact cells with
 // vector loads/stores; a workgroup covers 256*CPT cells of one instance.
-// The population arrays are padded to a multiple of 1024 cells, so a vector
-// never leaves its array; padding cells keep their rest values.
+// The populations are padded to a multiple of 1024 cells and stored in 128-cell
+// blocks, so a vector never leaves its block; padding cells keep their rest
+// values.
 // ---------------------------------------------------------------------------
 template<class T, int N> struct VecT { typedef T type __attribute__((ext_vector_type(N))); };
 template<class T> struct VecT<T, 1> { typedef T type; };
@@ -230,8 +232,6 @@ k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         const int vslot = st[w.inst].vel_next;
         const InstDesc &D = insts[w.inst];
         gptr<T> f = as_global<T>(D.f);
-        const int64_t stride = D.stride;
-
         const int c0 = (int)threadIdx.x * CPT;
         const int nreal = w.ncell - c0;             // real cells of this thread (may be <= 0)
         const int64_t i = w.first + c0;
@@ -239,7 +239,7 @@ k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         if(nreal > 0) {
             V in[L::Q];
 #pragma unroll
-            for(int k = 0; k < L::Q; k++) in[k] = ld<NT>((gptr<V>)(f + k * stride + i));
+            for(int k = 0; k < L::Q; k++) in[k] = ld<NT>((gptr<V>)(f + pop_at<L::Q>(k, (uint32_t)i)));
             V out[L::Q];
 #pragma unroll
             for(int c = 0; c < CPT; c++) {
@@ -253,7 +253,7 @@ k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
                     lane_of<T, CPT>(out[L::opp(k)], c) = real ? v[k] : lane_of<T, CPT>(in[L::opp(k)], c);
             }
 #pragma unroll
-            for(int k = 0; k < L::Q; k++) stv<NT>((gptr<V>)(f + k * stride + i), out[k]);
+            for(int k = 0; k < L::Q; k++) stv<NT>((gptr<V>)(f + pop_at<L::Q>(k, (uint32_t)i)), out[k]);
         }
         if(vslot && nreal > 0) {
 #pragma unroll
@@ -327,7 +327,6 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         const int vslot = st[w.inst].vel_next;
         const InstDesc &D = insts[w.inst];
         gptr<T> f = as_global<T>(D.f);
-        const uint32_t stride = (uint32_t)D.stride;    // Q * stride < 2^32 (checked on the host)
         const int ny = D.ny, nseg = D.nseg;
         cptr<SegInfo> seg = as_const<SegInfo>(D.seg);
 
@@ -362,14 +361,13 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
                 Sk[k] = L::ex(k + 1) != 0 ? rs[k] : 0u;
             }
             uint32_t off[Q];
-            off[0] = i;
+            off[0] = pop_at<Q>(0, i);
 #pragma unroll
             for(int k = 1; k < Q; k++) {
-                const uint32_t add = (uint32_t)L::opp(k) * stride;
-                uint32_t nbi = lane_rank_plus(Mk[k - 1], Bk[k - 1] + add);
-                if(L::ex(k) > 0) nbi = wrapL ? Sk[k - 1] + add : nbi;
-                if(L::ex(k) < 0) nbi = wrapR ? Sk[k - 1] + add : nbi;
-                off[k] = lane_select(Mk[k - 1], nbi, i + (uint32_t)k * stride);
+                uint32_t nbi = lane_rank_plus(Mk[k - 1], Bk[k - 1]);
+                if(L::ex(k) > 0) nbi = wrapL ? Sk[k - 1] : nbi;
+                if(L::ex(k) < 0) nbi = wrapR ? Sk[k - 1] : nbi;
+                off[k] = lane_select(Mk[k - 1], pop_at<Q>(L::opp(k), nbi), pop_at<Q>(k, i));
             }
             T v[Q];
 #pragma unroll
@@ -413,7 +411,6 @@ k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         const int vslot = st[w.inst].vel_next;
         const InstDesc &D = insts[w.inst];
         gptr<T> f = as_global<T>(D.f);
-        const uint32_t stride = (uint32_t)D.stride;    // Q * stride < 2^32 (checked on the host)
         const int c = (int)threadIdx.x;
         const bool act = c < w.ncell;
         const uint32_t i = (uint32_t)(w.first + c);
@@ -451,11 +448,11 @@ k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
                 for(int k = 0; k < NK; k++) rb[k] = b[k];
             }
             uint32_t off[Q];
-            off[0] = i;
+            off[0] = pop_at<Q>(0, i);
 #pragma unroll
             for(int k = 1; k < Q; k++) {
                 const uint32_t rel = r.c.rel[k - 1];
-                off[k] = rel == CREL_SOLID ? i + (uint32_t)k * stride : rb[k - 1] + rel;
+                off[k] = rel == CREL_SOLID ? pop_at<Q>(k, i) : pop_at<Q>(L::opp(k), rb[k - 1] + rel);
             }
             T v[Q];
 #pragma unroll

@@ -153,6 +153,22 @@ FP_HD void fix_split(double u, double &c0, double &c1, double &c2, int &bad)
 // ---------------------------------------------------------------------------
 // device descriptors
 // ---------------------------------------------------------------------------
+// Population storage.  Fluid cells are numbered in compact order; the cells
+// are grouped in blocks of 128 and a block stores population 0 of its 128
+// cells, then population 1, ..., then Q-1 (an array of structures of arrays).
+// A step kernel's wavefront then streams one contiguous 19 x 512-byte region
+// (FP32) instead of 19 arrays far apart: the local step runs at 5.9-6.0 TB/s
+// instead of 5.6, as a copy over the two layouts does (6.3 against 5.7).
+// The arrays hold a whole number of 1024-cell chunks, so blocks never split.
+constexpr int POP_BLOCK_LOG2 = 7;
+// the element index of population k of compact cell n (Q * stride < 2^32)
+template<int Q> FP_HD uint32_t pop_at(int k, uint32_t n)
+{
+    return (n >> POP_BLOCK_LOG2) * (uint32_t)(Q << POP_BLOCK_LOG2) + ((uint32_t)k << POP_BLOCK_LOG2) +
+           (n & ((1u << POP_BLOCK_LOG2) - 1u));
+}
+
+// ---------------------------------------------------------------------------
 // one 64-cell run of a lattice row along x: fluid bits and the compact index
 // of its first fluid cell.  Rows are ordered z-major then y, x fastest, the
 // .raw order, so compact indices follow the reference's node order.
@@ -179,9 +195,9 @@ template<int NK> struct LinkRec {
 // fluid cells per wavefront), one record per fluid cell: for every direction
 // k >= 1, the neighbour at cell - e_k as its offset inside its own row of
 // fluid cells (0xFFFF: solid, the link bounces back), and the cell's row.
-// Per row, rowbase[row * (Q-1) + k - 1] = opp(k) * stride + the compact index
-// of the first fluid cell of that neighbour row, so the neighbour's slot is
-// rowbase + rel.  Records are padded to 16 bytes for vector loads.
+// Per row, rowbase[row * (Q-1) + k - 1] = the compact index of the first fluid
+// cell of that neighbour row, so the neighbour is compact cell rowbase + rel.
+// Records are padded to 16 bytes for vector loads.
 constexpr uint16_t CREL_SOLID = 0xFFFF;
 template<int NK> struct alignas(16) CRec {
     uint16_t rel[NK];
@@ -200,9 +216,9 @@ struct InstDesc {
     const void *ctab;       // CRec<Q-1>[n_fluid] (compact neighbour step), or null
     const uint32_t *rowbase;   // [nrows * (Q-1)] (compact neighbour step)
     const uint32_t *rowspec;   // [nrows * (Q-1)]
-    void    *f;             // populations f[k * stride + i]
+    void    *f;             // populations, f[pop_at<Q>(k, i)]
     float   *vel[2];        // compact velocity fields (DIM floats per cell), or null
-    int64_t  stride;        // elements per population array
+    int64_t  stride;        // padded cell count (a multiple of 1024): Q * stride elements
     int64_t  n_fluid;
     int      nx, ny, nz, nseg;
     double   F[3];          // body force
