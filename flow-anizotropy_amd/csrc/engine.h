@@ -26,8 +26,8 @@ struct Geometry {
     SegInfo *d_seg = nullptr;
     void *d_links = nullptr;         // LinkRec<Q-1> per segment (neighbour-step links)
     uint32_t *d_rowspec = nullptr;   // wrap-lane neighbour index per row and direction
-    void *d_ctab = nullptr;          // CRec<Q-1> per fluid cell (built on first use)
-    uint32_t *d_rowbase = nullptr;   // compact-step row bases
+    void *d_ctab = nullptr;          // CRec per fluid cell (built on first use)
+    uint32_t *d_rowbase = nullptr;   // compact-step row table (start, last rank per slot)
     size_t links_bytes = 0, rowspec_bytes = 0, ctab_bytes = 0;
     std::vector<uint8_t> solid;      // host copy, x fastest
     int refs = 0;                    // jobs using it

@@ -29,53 +29,59 @@ Geometry::~Geometry()
 
 size_t Geometry::compact_bytes() const
 {
-    const size_t rec = dim == 3 ? sizeof(CRec<18>) : sizeof(CRec<8>);
-    const int nk = dim == 3 ? 18 : 8;
-    return (size_t)n_fluid * rec + (size_t)nrows * nk * sizeof(uint32_t);
+    const size_t rec = dim == 3 ? sizeof(CRec<RowSlots<D3Q19>::N>) : sizeof(CRec<RowSlots<D2Q9>::N>);
+    const int words = dim == 3 ? rowtab_words<D3Q19>() : rowtab_words<D2Q9>();
+    return (size_t)n_fluid * rec + (size_t)nrows * words * sizeof(uint32_t);
 }
 
 static int64_t round_up(int64_t v, int64_t m);
 
-// The compact neighbour table (CRec per fluid cell, row bases per row) from
-// the segment table.  False when a row holds 0xFFFF or more fluid cells (the
-// offsets are 16-bit); the row-segment neighbour step is used then.
+// The compact neighbour table (CRec per fluid cell, start and last rank of
+// every row slot per row) from the segment table.  False when a row holds
+// 0xFFFF or more cells (the ranks are 16-bit); the row-segment neighbour step
+// is used then.
 template<class L>
 static bool build_compact(const Geometry &g, const std::vector<SegInfo> &h,
-                          std::vector<CRec<L::Q - 1>> &tab, std::vector<uint32_t> &rowbase)
+                          std::vector<CRec<RowSlots<L>::N>> &tab, std::vector<uint32_t> &rowtab)
 {
-    constexpr int NK = L::Q - 1;
+    constexpr int NR = RowSlots<L>::N;
+    const int NW = rowtab_words<L>();
     if(g.nx >= 0xFFFF) return false;
     auto wrap = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
-    auto rowstart = [&](int64_t r) { return h[(size_t)(r * g.nseg)].start; };
-    tab.assign((size_t)g.n_fluid, CRec<NK>{});
-    rowbase.assign((size_t)g.nrows * NK, 0u);
+    auto rowstart = [&](int64_t r) -> uint32_t { return h[(size_t)(r * g.nseg)].start; };
+    auto rowcount = [&](int64_t r) -> uint32_t {
+        return (uint32_t)((r + 1 < g.nrows ? (int64_t)rowstart(r + 1) : g.n_fluid) - rowstart(r));
+    };
+    tab.assign((size_t)g.n_fluid, CRec<NR>{});
+    rowtab.assign((size_t)g.nrows * NW, 0u);
     for(int64_t row = 0; row < g.nrows; row++) {
         const int y = L::DIM == 3 ? (int)(row % g.ny) : (int)row;
         const int z = L::DIM == 3 ? (int)(row / g.ny) : 0;
-        int64_t rr[L::Q];
-        for(int k = 1; k < L::Q; k++) {
-            const int yy = wrap(y - L::ey(k), g.ny);
-            rr[k] = L::DIM == 3 ? (int64_t)wrap(z - L::ez(k), g.nz) * g.ny + yy : yy;
-            rowbase[(size_t)row * NK + k - 1] = (uint32_t)rowstart(rr[k]);
+        const uint8_t *rs[NR];
+        for(int m = 0; m < NR; m++) {
+            const int yy = wrap(y - RowSlots<L>::ey(m), g.ny);
+            const int64_t r = L::DIM == 3 ? (int64_t)wrap(z - RowSlots<L>::ez(m), g.nz) * g.ny + yy : yy;
+            rs[m] = g.solid.data() + r * g.nx;
+            const uint32_t cnt = rowcount(r);
+            rowtab[(size_t)row * NW + m] = rowstart(r);
+            rowtab[(size_t)row * NW + NR + m] = cnt ? cnt - 1u : 0u;
         }
-        const uint8_t *own = g.solid.data() + row * g.nx;
+        uint32_t P[NR] = {};                  // fluid cells of each slot's row before x
         uint32_t i = rowstart(row);
         for(int x = 0; x < g.nx; x++) {
-            if(own[x]) continue;
-            CRec<NK> &R = tab[i];
-            R.row = (uint32_t)row;
-            for(int k = 1; k < L::Q; k++) {
-                const int xx = wrap(x - L::ex(k), g.nx);
-                const uint8_t *nb = g.solid.data() + rr[k] * g.nx;
-                if(nb[xx]) { R.rel[k - 1] = CREL_SOLID; continue; }
-                // fluid cells of the neighbour row before xx
-                const int sg = xx >> 6;
-                const SegInfo &s = h[(size_t)(rr[k] * g.nseg + sg)];
-                const uint64_t below = (xx & 63) ? (~0ull >> (64 - (xx & 63))) : 0ull;
-                const uint32_t idx = s.start + (uint32_t)__builtin_popcountll(s.mask & below);
-                R.rel[k - 1] = (uint16_t)(idx - rowstart(rr[k]));
+            if(!rs[0][x]) {
+                CRec<NR> &R = tab[i++];
+                R.row = (uint32_t)row;
+                uint32_t flags = 0;
+                for(int m = 0; m < NR; m++)
+                    for(int e = 0; e < 3; e++)
+                        if(!rs[m][wrap(x - 1 + e, g.nx)]) flags |= 1u << (3 * m + e);
+                if(x == 0) flags |= CFLAG_X0;
+                if(x == g.nx - 1) flags |= CFLAG_XLAST;
+                R.flags = flags;
+                for(int m = 1; m < NR; m++) R.rank[m - 1] = (uint16_t)P[m];
             }
-            i++;
+            for(int m = 0; m < NR; m++) P[m] += rs[m][x] == 0;
         }
     }
     return true;
@@ -253,10 +259,11 @@ static int64_t verify_links(const Geometry &g, const uint8_t *solid, const std::
 template<class L>
 static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std::vector<SegInfo> &h)
 {
-    constexpr int NK = L::Q - 1;
-    std::vector<CRec<NK>> tab;
-    std::vector<uint32_t> rowbase;
-    if(!build_compact<L>(g, h, tab, rowbase)) return 0;
+    constexpr int NR = RowSlots<L>::N;
+    const int NW = rowtab_words<L>();
+    std::vector<CRec<NR>> tab;
+    std::vector<uint32_t> rowtab;
+    if(!build_compact<L>(g, h, tab, rowtab)) return 0;
     std::vector<int64_t> cidx((size_t)g.nn, -1);
     int64_t c = 0;
     for(int64_t n = 0; n < g.nn; n++) if(solid[n] == 0) cidx[(size_t)n] = c++;
@@ -268,13 +275,19 @@ static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std
         const int64_t row = n / g.nx;
         const int y = L::DIM == 3 ? (int)(row % g.ny) : (int)row;
         const int z = L::DIM == 3 ? (int)(row / g.ny) : 0;
-        const CRec<NK> &R = tab[(size_t)i];
+        const CRec<NR> &R = tab[(size_t)i];
         if((int64_t)R.row != row) { bad++; continue; }
+        // the kernel's decoding of the record
+        uint32_t P[NR], start[NR], last[NR], off[L::Q];
+        for(int m = 0; m < NR; m++) {
+            start[m] = rowtab[(size_t)row * NW + m];
+            last[m] = rowtab[(size_t)row * NW + NR + m];
+        }
+        P[0] = (uint32_t)i - start[0];
+        for(int m = 1; m < NR; m++) P[m] = R.rank[m - 1];
+        compact_offsets<L>((uint32_t)i, R.flags, P, start, last, off);
+        if(off[0] != pop_at<L::Q>(0, (uint32_t)i)) bad++;
         for(int k = 1; k < L::Q; k++) {
-            const uint16_t rel = R.rel[k - 1];
-            const uint32_t off = rel == CREL_SOLID
-                                     ? pop_at<L::Q>(k, (uint32_t)i)
-                                     : pop_at<L::Q>(L::opp(k), rowbase[(size_t)row * NK + k - 1] + rel);
             const int xx = (x - L::ex(k) + g.nx) % g.nx;
             const int yy = (y - L::ey(k) + g.ny) % g.ny;
             const int zz = L::DIM == 3 ? (z - L::ez(k) + g.nz) % g.nz : 0;
@@ -282,7 +295,7 @@ static int64_t verify_compact(const Geometry &g, const uint8_t *solid, const std
             const uint32_t want = solid[(size_t)dn]
                                       ? pop_at<L::Q>(k, (uint32_t)i)
                                       : pop_at<L::Q>(L::opp(k), (uint32_t)cidx[(size_t)dn]);
-            if(off != want) bad++;
+            if(off[k] != want) bad++;
         }
     }
     return bad;
@@ -309,8 +322,8 @@ std::string geometry_build_compact(Geometry &g)
     std::vector<uint32_t> rowbase;
     size_t tab_bytes = 0;
     const void *tab_data = nullptr;
-    std::vector<CRec<18>> t3;
-    std::vector<CRec<8>> t2;
+    std::vector<CRec<RowSlots<D3Q19>::N>> t3;
+    std::vector<CRec<RowSlots<D2Q9>::N>> t2;
     bool ok;
     if(g.dim == 3) { ok = build_compact<D3Q19>(g, h, t3, rowbase); tab_bytes = t3.size() * sizeof(t3[0]); tab_data = t3.data(); }
     else           { ok = build_compact<D2Q9>(g, h, t2, rowbase); tab_bytes = t2.size() * sizeof(t2[0]); tab_data = t2.data(); }

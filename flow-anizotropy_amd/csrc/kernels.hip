@@ -389,8 +389,9 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 
 // ---------------------------------------------------------------------------
 // neighbour step, compact form (even t): 64 consecutive fluid cells per
-// wavefront, whatever rows they lie in.  Each cell reads its CRec (16-bit
-// offsets of its 18 neighbours inside their rows); the row bases come from
+// wavefront, whatever rows they lie in.  Each cell reads its 24-byte CRec
+// (fluid bits at x - 1, x, x + 1 and the rank of x in each neighbour row);
+// the row table (start and last rank of every neighbour row) comes from
 // scalar loads when the wavefront's cells lie in at most two rows (the usual
 // case), else from vector loads.  The same slots as k_nbr are read and
 // written, so the two forms give identical results.
@@ -417,11 +418,13 @@ k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 
         double u[3] = {0.0, 0.0, 0.0};
         if(act) {
-            // the cell's record, as 16-byte vector loads
-            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-            constexpr int NV = (int)(sizeof(CRec<NK>) / 16);
-            union { u4 v[NV]; CRec<NK> c; } r;
-            gptr<const u4> src = as_global<const u4>(const_cast<void *>(D.ctab)) + (int64_t)i * NV;
+            // the cell's record, as 8-byte vector loads
+            constexpr int NR = RowSlots<L>::N;
+            constexpr int NW = 2 * NR;
+            typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+            constexpr int NV = (int)(sizeof(CRec<NR>) / 8);
+            union { u2 v[NV]; CRec<NR> c; } r;
+            gptr<const u2> src = as_global<const u2>(const_cast<void *>(D.ctab)) + (int64_t)i * NV;
 #pragma unroll
             for(int q = 0; q < NV; q++) r.v[q] = src[q];
             const uint32_t row = r.c.row;
@@ -431,29 +434,30 @@ k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
             const uint32_t row0 = uni(row);
             const uint32_t row1 = __builtin_amdgcn_readlane(row, last);
             const bool two = __ballot(row != row0 && row != row1) == 0ull;
-            uint32_t rb[NK];
+            uint32_t tb[NW];
             if(two) {
-                cptr<uint32_t> b0 = as_const<uint32_t>(D.rowbase) + (int64_t)row0 * NK;
-                cptr<uint32_t> b1 = as_const<uint32_t>(D.rowbase) + (int64_t)row1 * NK;
+                cptr<uint32_t> b0 = as_const<uint32_t>(D.rowbase) + (int64_t)row0 * NW;
+                cptr<uint32_t> b1 = as_const<uint32_t>(D.rowbase) + (int64_t)row1 * NW;
                 const bool in0 = row == row0;
 #pragma unroll
-                for(int k = 0; k < NK; k++) {
-                    const uint32_t a0 = b0[k], a1 = b1[k];
-                    rb[k] = in0 ? a0 : a1;
+                for(int w = 0; w < NW; w++) {
+                    const uint32_t a0 = b0[w], a1 = b1[w];
+                    tb[w] = in0 ? a0 : a1;
                 }
             } else {
                 gptr<const uint32_t> b = as_global<const uint32_t>((void *)const_cast<uint32_t *>(D.rowbase)) +
-                                         (int64_t)row * NK;
+                                         (int64_t)row * NW;
 #pragma unroll
-                for(int k = 0; k < NK; k++) rb[k] = b[k];
+                for(int w = 0; w < NW; w++) tb[w] = b[w];
             }
+            uint32_t rk[NR], start[NR], lastr[NR];
+#pragma unroll
+            for(int m = 0; m < NR; m++) { start[m] = tb[m]; lastr[m] = tb[NR + m]; }
+            rk[0] = i - start[0];
+#pragma unroll
+            for(int m = 1; m < NR; m++) rk[m] = r.c.rank[m - 1];
             uint32_t off[Q];
-            off[0] = pop_at<Q>(0, i);
-#pragma unroll
-            for(int k = 1; k < Q; k++) {
-                const uint32_t rel = r.c.rel[k - 1];
-                off[k] = rel == CREL_SOLID ? pop_at<Q>(k, i) : pop_at<Q>(L::opp(k), rb[k - 1] + rel);
-            }
+            compact_offsets<L>(i, r.c.flags, rk, start, lastr, off);
             T v[Q];
 #pragma unroll
             for(int k = 0; k < Q; k++) v[k] = ld<false>(f + off[k]);

@@ -191,18 +191,64 @@ template<int NK> struct LinkRec {
     uint32_t base[NK];
 };
 
-// Compact neighbour table (the alternative neighbour step that packs 64
-// fluid cells per wavefront), one record per fluid cell: for every direction
-// k >= 1, the neighbour at cell - e_k as its offset inside its own row of
-// fluid cells (0xFFFF: solid, the link bounces back), and the cell's row.
-// Per row, rowbase[row * (Q-1) + k - 1] = the compact index of the first fluid
-// cell of that neighbour row, so the neighbour is compact cell rowbase + rel.
-// Records are padded to 16 bytes for vector loads.
-constexpr uint16_t CREL_SOLID = 0xFFFF;
-template<int NK> struct alignas(16) CRec {
-    uint16_t rel[NK];
-    uint32_t row;
+// Compact neighbour table (the neighbour step that packs 64 fluid cells per
+// wavefront), one record per fluid cell.  A neighbour x - ex of row r' (the
+// row at y - ey, z - ez) has compact index start(r') + its rank among the
+// fluid cells of r'.  With P = the rank of x in r' (fluid cells of r' before
+// x), that rank is P for ex = 0, P - 1 for x - 1 (when fluid) and
+// P + fluid(x) for x + 1, and across the periodic x wrap the last rank of r'
+// (x - 1 from x = 0) or 0 (x + 1 from x = nx - 1).  So a record keeps, per
+// row slot (0 = the cell's own row, then the 8 rows at (ey, ez) != (0, 0) in
+// 3D, 2 in 2D), three fluid bits (x - 1, x, x + 1) and the rank P, and per
+// lattice row and slot a table holds start(r') and the last rank of r':
+// 24 bytes per cell for D3Q19 instead of one 16-bit offset per direction.
+template<class L> struct RowSlots {
+    static constexpr int N = L::DIM == 3 ? 9 : 3;
+    // the row slot direction k reads from
+    static FP_HD int of(int k)
+    {
+        constexpr int py[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+        constexpr int pz[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+        for(int m = 0; m < 9; m++)
+            if(py[m] == L::ey(k) && pz[m] == (L::DIM == 3 ? L::ez(k) : 0)) return m;
+        return -1;
+    }
+    static FP_HD int ey(int m) { constexpr int py[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1}; return py[m]; }
+    static FP_HD int ez(int m) { constexpr int pz[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1}; return pz[m]; }
 };
+constexpr uint32_t CFLAG_X0 = 1u << 30;      // the cell is at x = 0
+constexpr uint32_t CFLAG_XLAST = 1u << 31;   // the cell is at x = nx - 1
+template<int NR> struct alignas(8) CRec {
+    uint32_t row;             // the cell's lattice row
+    uint32_t flags;           // bit 3m + e: fluid at x - 1 + e in row slot m; CFLAG_X0, CFLAG_XLAST
+    uint16_t rank[NR - 1];    // P of row slots 1..NR-1 (slot 0: the cell's own rank)
+};
+// per lattice row: start(r') of every slot, then the last rank of every slot
+template<class L> FP_HD int rowtab_words() { return 2 * RowSlots<L>::N; }
+
+// The slots a compact-form cell reads population k from (and writes opp(k)
+// to): pop_at(opp k, neighbour) when the neighbour at cell - e_k is fluid,
+// else (halfway bounce-back) pop_at(k, i).  P[m]: the rank of x in slot m's
+// row (P[0]: the cell's own); start/last: the row table entries of its row.
+template<class L>
+FP_HD void compact_offsets(uint32_t i, uint32_t flags, const uint32_t (&P)[RowSlots<L>::N],
+                           const uint32_t (&start)[RowSlots<L>::N],
+                           const uint32_t (&last)[RowSlots<L>::N], uint32_t (&off)[L::Q])
+{
+    off[0] = pop_at<L::Q>(0, i);
+    const bool x0 = (flags & CFLAG_X0) != 0, xl = (flags & CFLAG_XLAST) != 0;
+#pragma unroll
+    for(int k = 1; k < L::Q; k++) {
+        const int m = RowSlots<L>::of(k);
+        const int e = 1 - L::ex(k);                      // x - ex = x - 1 + e
+        const bool fluid = (flags >> (3 * m + e)) & 1u;
+        uint32_t rank;
+        if(L::ex(k) == 0) rank = P[m];
+        else if(L::ex(k) > 0) rank = x0 ? last[m] : P[m] - 1u;
+        else rank = xl ? 0u : P[m] + ((flags >> (3 * m + 1)) & 1u);
+        off[k] = fluid ? pop_at<L::Q>(L::opp(k), start[m] + rank) : pop_at<L::Q>(k, i);
+    }
+}
 
 enum Prec : int {
     PREC_F64 = 0,       // f in double, the reference's arithmetic
@@ -213,8 +259,8 @@ enum Prec : int {
 struct InstDesc {
     const SegInfo *seg;     // [nrows * nseg]
     const void *links;      // LinkRec<Q-1>[nrows * nseg]
-    const void *ctab;       // CRec<Q-1>[n_fluid] (compact neighbour step), or null
-    const uint32_t *rowbase;   // [nrows * (Q-1)] (compact neighbour step)
+    const void *ctab;       // CRec[n_fluid] (compact neighbour step), or null
+    const uint32_t *rowbase;   // [nrows * rowtab_words] (compact neighbour step)
     const uint32_t *rowspec;   // [nrows * (Q-1)]
     void    *f;             // populations, f[pop_at<Q>(k, i)]
     float   *vel[2];        // compact velocity fields (DIM floats per cell), or null
@@ -481,11 +527,12 @@ struct Tuning {
                             // 2 = per instance by porosity (COMPACT_BELOW_POROSITY)
 };
 
-// Below this porosity the compact neighbour step is the faster form: per fluid
-// cell its cost is flat, the row-segment step's grows as 1/porosity (measured
-// on MI355X, FP32 D3Q19: 48.6 ps/cell compact at porosity 0.62 and 0.14;
-// row segments 43.3 ps at 0.62, 82.6 ps at 0.14; equal near 0.42).
-constexpr double COMPACT_BELOW_POROSITY = 0.42;
+// Below this porosity the compact neighbour step is the faster form: it keeps
+// every lane busy, while the row-segment step idles the lanes of solid cells.
+// Compact time over row-segment time on MI355X (D3Q19, 192^3 ellipsoids,
+// FP32 / FP64): 0.68 / 0.71 at porosity 0.30, 0.89 / 0.94 at 0.52, 0.94 / 0.97
+// at 0.62, 1.02 / 1.04 at 0.72, 1.05 / 1.06 at 0.92.
+constexpr double COMPACT_BELOW_POROSITY = 0.68;
 
 // work of one workgroup of the local (even A-A) kernel: a run of compact cells
 struct LocalWork {
