@@ -162,14 +162,15 @@ def test_now_is_monotonic(lib):
 
 def test_neighbour_link_tables_match_a_dense_lookup(lib):
     """the per-segment link records of the row-segment neighbour step and the
-    per-cell table of the compact one both give, for every fluid cell and
-    direction, the slot of the periodic neighbour at cell - e_k (or the cell's
-    own slot when that neighbour is solid): random lattices of every width
-    class (x-runs of 1, < 64, 64, > 64 cells) in 3D and 2D"""
+    per-cell records of the compact one (decoded by the kernel's own
+    compact_offsets) both give, for every fluid cell and direction, the slot of
+    the periodic neighbour at cell - e_k (or the cell's own slot when that
+    neighbour is solid): random lattices of every width class (x-runs of 1, 2,
+    < 64, 64, > 64 cells, where x - 1 and x + 1 wrap) in 3D and 2D"""
     from flowperm import _lib
     rng = np.random.default_rng(0)
-    shapes = [(1, 1, 1), (2, 3, 1), (3, 2, 63), (2, 2, 64), (3, 4, 65), (2, 3, 128), (5, 4, 130),
-              (9, 7, 11), (16, 16, 16)]
+    shapes = [(1, 1, 1), (2, 3, 1), (3, 3, 2), (3, 2, 63), (2, 2, 64), (3, 4, 65), (2, 3, 128),
+              (5, 4, 130), (9, 7, 11), (16, 16, 16)]
     for shp in shapes:
         for dim in (3, 2):
             s = shp if dim == 3 else shp[1:]
