@@ -264,9 +264,9 @@ typedef struct fp_tuning {
                                  (0: one workgroup per work item)                     */
     int32_t nbr_bpc;          /* the same for the neighbour kernel                    */
     int32_t nbr_mode;         /* neighbour step: 0 = one wavefront per 64-cell row
-                                 segment, 1 = 64 fluid cells per wavefront (a 48-byte
-                                 per-cell neighbour table; faster on sparse lattices),
-                                 2 = chosen per structure by porosity (default)       */
+                                 segment, 1 = 64 fluid cells per wavefront (a 24-byte
+                                 per-cell neighbour record; faster below porosity
+                                 ~0.7), 2 = chosen per structure by porosity (default) */
 } fp_tuning;
 
 /* Set / read the launch shapes a context uses for its solves and batches. */
