@@ -942,10 +942,11 @@ extern "C" int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_p
 // neighbour shape i mod FP_TUNE_NNBR (the kernels are timed independently by
 // events).
 // ---------------------------------------------------------------------------
-static const int TUNE_LOCAL[24][3] = {      // cells/thread, blocks/CU, non-temporal
-    {1, 2, 0}, {1, 4, 0}, {1, 8, 0}, {1, 16, 0}, {2, 2, 0}, {2, 4, 0}, {2, 8, 0}, {2, 16, 0},
-    {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {4, 16, 0}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {1, 16, 1},
-    {2, 2, 1}, {2, 4, 1}, {2, 8, 1}, {2, 16, 1}, {4, 2, 1}, {4, 4, 1}, {4, 8, 1}, {4, 16, 1}};
+// local step: cells/thread, blocks/CU (0: one per item), non-temporal
+static const int TUNE_LOCAL[24][3] = {
+    {1, 0, 0}, {1, 2, 0}, {1, 8, 0}, {1, 16, 0}, {2, 0, 0}, {2, 2, 0}, {2, 8, 0}, {2, 16, 0},
+    {4, 0, 0}, {4, 2, 0}, {4, 8, 0}, {4, 16, 0}, {1, 0, 1}, {1, 2, 1}, {1, 8, 1}, {1, 16, 1},
+    {2, 0, 1}, {2, 2, 1}, {2, 8, 1}, {2, 16, 1}, {4, 0, 1}, {4, 2, 1}, {4, 8, 1}, {4, 16, 1}};
 // neighbour step: segments per workgroup (row-segment form), blocks/CU (0:
 // one per item), form (0 row segments, 1 compact)
 static const int TUNE_NBR[FP_TUNE_NNBR][3] = {
