@@ -61,8 +61,13 @@ def assert_printed_close(ours, ref, floor=0.0, what=""):
     assert abs(ours - ref) <= tol, f"{what}: ours {ours!r} ref {ref!r} |d|={abs(ours-ref):.3e} tol {tol:.3e}"
 
 
-GOLDEN_FAST = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5"]
-GOLDEN_ALL = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5", "sph_x28y24z20_s1",
+# GOLDEN_FAST: also re-solved by the CPU oracle (seconds each); rows of 70 cells
+# (a segment boundary) and a porosity-0.3 field.  GOLDEN_ALL adds 130-cell rows
+# and the longer cases, compared with the reference rows only.
+GOLDEN_FAST = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5",
+               "ell_x70y12z10_fast_s6", "grf_x36y20z24_fast_s8"]
+GOLDEN_ALL = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5", "ell_x70y12z10_fast_s6",
+              "ell_x130y8z6_fast_s7", "grf_x36y20z24_fast_s8", "sph_x28y24z20_s1",
               "slit32_h10", "slit32_h20", "c1_64_s0", "c1_64_s1", "ell24_validate_s3"]
 
 

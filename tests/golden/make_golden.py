@@ -57,6 +57,16 @@ def percolating_ellipsoids(shape, seed, porosity):
         s += 1000
 
 
+def percolating_grf(shape, seed, porosity):
+    """A Gaussian random field reduced to a pore cluster that wraps along x, y and z."""
+    s = seed
+    while True:
+        m = masks.grf(shape, porosity, corr=(3.0, 3.0, 6.0), seed=s, require_all=True)
+        if all(masks.percolates(m)):
+            return m, s
+        s += 1000
+
+
 def case_list():
     """name -> (mask builder, flags).  Shapes are (nz, ny, nx)."""
     cases = {
@@ -74,6 +84,11 @@ def case_list():
         # velocity export (export_velocity, lbm3d-perm.cpp:305-317)
         "sph12_velocity_s4": (lambda: percolating_spheres((12, 12, 12), 4, rmin=2.0, rmax=3.0),
                               ["--velocity", "{prefix}"]),
+        # rows longer than one 64-cell segment (x-runs that cross segment
+        # boundaries and wrap), and a low-porosity field
+        "ell_x70y12z10_fast_s6": (lambda: percolating_ellipsoids((10, 12, 70), 6, 0.55), FAST),
+        "ell_x130y8z6_fast_s7": (lambda: percolating_ellipsoids((6, 8, 130), 7, 0.6), FAST),
+        "grf_x36y20z24_fast_s8": (lambda: percolating_grf((24, 20, 36), 8, 0.3), FAST),
         # the 64^3 parity workload: spheres r~U[3,6], porosity 0.6, F = 1e-5
         "c1_64_s0": (lambda: percolating_spheres((64, 64, 64), 0), C1),
         "c1_64_s1": (lambda: percolating_spheres((64, 64, 64), 1), C1),
