@@ -513,14 +513,14 @@ struct RedWork {
 // launch shapes of the step kernels (chosen by the autotuner)
 // The defaults are shapes the autotuner picks on MI355X for D3Q19 at 256^3 in
 // both precisions (local 2 cells/thread, non-temporal, one workgroup per item;
-// neighbour 8 stacked row segments, one workgroup per item): about 20% faster
+// neighbour 4 stacked row segments, one workgroup per item): about 20% faster
 // per step than 1 cell/thread with cached accesses and persistent grids.  With
 // populations in blocks, one workgroup per item beats 16 resident workgroups per
 // CU looping over items: local step 0.253 against 0.267 ms (FP32), 0.533
 // against 0.572 (FP64).
 struct Tuning {
     int local_cpt = 2;      // cells per thread in the local kernel: 1, 2 or 4
-    int nbr_waves = 8;      // wavefronts per workgroup in the neighbour kernel: 1..8, one
+    int nbr_waves = 4;      // wavefronts per workgroup in the neighbour kernel: 1..8, one
                             // row segment each, stacked along z (along y in 2D)
     int nontemporal = 1;    // local kernel: population loads/stores bypass cache allocation
     int local_bpc = 0;      // resident workgroups per CU launched (each loops over items);

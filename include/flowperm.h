@@ -252,7 +252,7 @@ typedef struct fp_bench_out {
 } fp_bench_out;
 
 /* Launch shapes of the step kernels; the autotuner picks them per device.  A new
-   context starts with local_cpt 2, nbr_waves 8, chunk_steps 200, nontemporal 1,
+   context starts with local_cpt 2, nbr_waves 4, chunk_steps 200, nontemporal 1,
    local_bpc 0, nbr_bpc 0, nbr_mode 2 (the shapes tuned for D3Q19 at 256^3). */
 typedef struct fp_tuning {
     int32_t local_cpt;        /* cells per thread, local kernel: 1, 2 or 4            */
