@@ -11,7 +11,9 @@ namespace fp {
 
 // kernels.hip
 void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
-void kern_step(int lattice, int prec, bool local, const Tuning &tu, const InstDesc *insts,
+// b32: every instance's population arrays span less than 4 GiB (32-bit byte
+// offsets in the neighbour kernels)
+void kern_step(int lattice, int prec, bool local, const Tuning &tu, bool b32, const InstDesc *insts,
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
 void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s);
 // after a step: exact flux sums and the convergence update of every instance
@@ -152,6 +154,7 @@ private:
     LocalWork *d_cwork_ = nullptr;           // compact neighbour-step items (mode 1)
     size_t cap_cwork_ = 0;
     int mode_ = 0;                           // forms in the tables: 0, 1 or 2 (both)
+    bool b32_ = true;                        // every instance's populations span < 4 GiB
     int n_cwork_ = 0;
     Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
     RedWork *d_lredw_ = nullptr, *d_nredw_ = nullptr;     // second-level reduction items

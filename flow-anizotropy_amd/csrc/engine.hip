@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 
 namespace fp {
@@ -520,6 +521,12 @@ std::string Batch::rebuild()
         n_mode[m]++;
     }
     mode_ = n_mode[1] == 0 ? 0 : (n_mode[0] == 0 ? 1 : 2);
+    // FLOWPERM_WIDE_OFFSETS=1 forces the 64-bit addressing kernels (the ones
+    // instances of 4 GiB and more use), so tests can compare the two
+    const char *wide = getenv("FLOWPERM_WIDE_OFFSETS");
+    b32_ = !(wide && wide[0] == '1');
+    for(const Job *j : jobs_)
+        if((uint64_t)lattice_q(lattice_) * (uint64_t)j->stride * elem_bytes(prec_) > 0xFFFFFFFFull) b32_ = false;
     std::vector<InstDesc> desc(n_inst_);
     std::vector<InstCfg> cfg(n_inst_);
     std::vector<InstState> state(n_inst_);
@@ -734,18 +741,18 @@ void Batch::launch_finish(bool local)
 void Batch::launch_step(bool local)
 {
     if(local) {
-        kern_step(lattice_, prec_, true, tune_, d_inst_, d_state_, d_lwork_, n_lwork_, d_lpart_, stream_);
+        kern_step(lattice_, prec_, true, tune_, b32_, d_inst_, d_state_, d_lwork_, n_lwork_, d_lpart_, stream_);
         return;
     }
     // the row-segment items, then the compact ones (their partial slots follow)
     Tuning tu = tune_;
     if(n_nwork_ > 0) {
         tu.nbr_mode = 0;
-        kern_step(lattice_, prec_, false, tu, d_inst_, d_state_, d_nwork_, n_nwork_, d_npart_, stream_);
+        kern_step(lattice_, prec_, false, tu, b32_, d_inst_, d_state_, d_nwork_, n_nwork_, d_npart_, stream_);
     }
     if(n_cwork_ > 0) {
         tu.nbr_mode = 1;
-        kern_step(lattice_, prec_, false, tu, d_inst_, d_state_, d_cwork_, n_cwork_,
+        kern_step(lattice_, prec_, false, tu, b32_, d_inst_, d_state_, d_cwork_, n_cwork_,
                   d_npart_ + n_nwork_, stream_);
     }
 }
