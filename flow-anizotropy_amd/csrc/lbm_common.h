@@ -538,17 +538,13 @@ struct Tuning {
 // at 0.62, 1.02 / 1.04 at 0.72, 1.05 / 1.06 at 0.92.
 constexpr double COMPACT_BELOW_POROSITY = 0.68;
 
-// work of one workgroup of the local (even A-A) kernel: a run of compact cells
+// work of one workgroup of the local (in-place collide) step: a run of compact cells
 struct LocalWork {
     int32_t inst;
     int32_t ncell;
     int64_t first;
 };
 
-// work of one workgroup of the neighbour (odd A-A) kernel: nunit (row,
-// segment) units of one instance, one wavefront each, at first + w * ustride
-// (ustride = one row up in z, so the rows that share cache lines through the
-// z-links are processed together)
 // One neighbour-step work item: nunit row segments stacked along the slowest
 // axis, wave w taking the one at plane p0 + w.  3D: rows (y, z = p0 + w);
 // 2D: rows y = p0 + w.  seg is the 64-cell segment of the row.
