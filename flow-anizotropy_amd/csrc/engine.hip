@@ -564,14 +564,27 @@ std::string Batch::rebuild()
         }
         desc[i].np0 = (int32_t)nw.size();
         {
-            // slabs of nbr_waves rows stacked along z (2D: y); inside a slab
-            // y then the segment vary fastest
+            // slabs of nbr_waves rows stacked along z (2D: y); inside a slab y
+            // then the segment vary fastest.  The kernels split their items
+            // into 8 contiguous ranges, one per XCD.  Large lattices deal the
+            // rows in 8 bands of ny/8 rows, each band through all z, slab after
+            // slab: an XCD then owns one band and walks it in z, so the rows
+            // either side of a slab boundary are in flight together and share
+            // its L2 (256^3 FP32 step 0.318 against 0.332 ms; bands of 8-64
+            // rows inside per-XCD z ranges gave 0.327-0.331).  Below ~400 MB of
+            // populations the plain z order stays: an XCD's neighbour-step
+            // range then covers the same cells as its local-step range, and
+            // the L2 lines the one kernel leaves are the other's (192^3 FP32:
+            // bands 0.139, z order 0.135 ms; 96^3: 0.030 against 0.027).
             const int64_t nwv = tune_.nbr_waves;
             const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
             const int64_t ys = g.dim == 3 ? g.ny : 1;
+            const bool banded = (double)lattice_q(lattice_) * (double)j->stride * (double)elem_bytes(prec_) >= 4e8;
+            const int64_t band = banded ? (ys + 7) / 8 : ys;
+            for(int64_t y0 = 0; y0 < ys; y0 += band)
             for(int64_t p0 = 0; p0 < planes; p0 += nwv) {
                 const int nu = (int)std::min<int64_t>(nwv, planes - p0);
-                for(int64_t y = 0; y < ys; y++)
+                for(int64_t y = y0; y < std::min<int64_t>(ys, y0 + band); y++)
                     for(int sg = 0; sg < g.nseg; sg++) {
                         NbrWork w;
                         w.inst = i;
