@@ -32,6 +32,7 @@ struct Geometry {
     uint32_t *d_rowbase = nullptr;   // compact-step row table (start, last rank per slot)
     size_t links_bytes = 0, rowspec_bytes = 0, ctab_bytes = 0;
     std::vector<uint8_t> solid;      // host copy, x fastest
+    std::vector<uint32_t> row_first; // compact index of each row's first fluid cell; [nrows] = n_fluid
     int refs = 0;                    // jobs using it
     ~Geometry();
     size_t device_bytes() const

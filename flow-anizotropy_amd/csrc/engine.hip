@@ -208,6 +208,9 @@ std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, i
     std::string le = dim == 3 ? upload_links<D3Q19>(g, h) : upload_links<D2Q9>(g, h);
     if(!le.empty()) return le;
     g.solid.assign(solid, solid + g.nn);
+    g.row_first.resize((size_t)g.nrows + 1);
+    for(int64_t r = 0; r < g.nrows; r++) g.row_first[(size_t)r] = h[(size_t)(r * g.nseg)].start;
+    g.row_first[(size_t)g.nrows] = (uint32_t)g.n_fluid;
     return "";
 }
 
@@ -548,16 +551,36 @@ std::string Batch::rebuild()
             lw.push_back(w);
         }
         desc[i].lp1 = (int32_t)lw.size();
+        // large lattices deal their neighbour-step items to the XCDs in y bands
+        // (see below)
+        const bool banded = g.dim == 3 &&
+            (double)lattice_q(lattice_) * (double)j->stride * (double)elem_bytes(prec_) >= 4e8;
         if(j->nbr_mode == 1) {
-            // compact neighbour step: 256 consecutive fluid cells per item; its
-            // partial slots follow the row-segment ones (offset fixed below)
+            // compact neighbour step: up to 256 consecutive fluid cells per
+            // item; its partial slots follow the row-segment ones (offset
+            // fixed below).  Banded: the cells of rows y0..y1-1 of one plane
+            // are consecutive, so band b is plane after plane of such runs
+            // (256^3 FP32 0.385 against 0.406 ms, FP64 0.653 against 0.761;
+            // 512^3 GRF FP32 0.700 against 0.712).
             desc[i].np0 = (int32_t)cw.size();
-            for(int64_t c = 0; c < g.n_fluid; c += 256) {
-                LocalWork w;
-                w.inst = i;
-                w.ncell = (int32_t)std::min<int64_t>(256, g.n_fluid - c);
-                w.first = c;
-                cw.push_back(w);
+            auto runs = [&](int64_t a0, int64_t a1) {
+                for(int64_t c = a0; c < a1; c += 256) {
+                    LocalWork w;
+                    w.inst = i;
+                    w.ncell = (int32_t)std::min<int64_t>(256, a1 - c);
+                    w.first = c;
+                    cw.push_back(w);
+                }
+            };
+            if(banded) {
+                const int64_t band = (g.ny + 7) / 8;
+                for(int64_t y0 = 0; y0 < g.ny; y0 += band) {
+                    const int64_t y1 = std::min<int64_t>(g.ny, y0 + band);
+                    for(int64_t z = 0; z < g.nz; z++)
+                        runs(g.row_first[(size_t)(z * g.ny + y0)], g.row_first[(size_t)(z * g.ny + y1)]);
+                }
+            } else {
+                runs(0, g.n_fluid);
             }
             desc[i].np1 = (int32_t)cw.size();
             continue;
@@ -579,7 +602,6 @@ std::string Batch::rebuild()
             const int64_t nwv = tune_.nbr_waves;
             const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
             const int64_t ys = g.dim == 3 ? g.ny : 1;
-            const bool banded = (double)lattice_q(lattice_) * (double)j->stride * (double)elem_bytes(prec_) >= 4e8;
             const int64_t band = banded ? (ys + 7) / 8 : ys;
             for(int64_t y0 = 0; y0 < ys; y0 += band)
             for(int64_t p0 = 0; p0 < planes; p0 += nwv) {

@@ -30,13 +30,32 @@ def lattice256():
                             seed=0)
 
 
-def test_256cube_fp64_steps_bit_exact(gpu_ctx, oracle, lattice256):
-    from flowperm import solver
-    cfg = dict(ff=40, lag=2, min_steps=10 ** 6, max_steps=6, avg_steps=0)
-    F = (1e-5, 0.0, 0.0)
-    print(f"256^3 FP64 6 steps, porosity {1 - lattice256.mean():.4f}", flush=True)
-    g = solver.solve_direction3d(lattice256, F, ctx=gpu_ctx, velocity=True, precision="f64", **cfg)
-    o = oracle.solve_dir3d(lattice256, F, velocity=True, **cfg)
+STEPS6 = dict(ff=40, lag=2, min_steps=10 ** 6, max_steps=6, avg_steps=0)
+F6 = (1e-5, 0.0, 0.0)
+
+
+@pytest.fixture(scope="module")
+def oracle256(oracle, lattice256):
+    return oracle.solve_dir3d(lattice256, F6, velocity=True, **STEPS6)
+
+
+# both neighbour-step forms: at this size (>= 400 MB of populations) their work
+# items are dealt to the XCDs in y bands
+@pytest.mark.parametrize("mode", [0, 1], ids=["row_segments", "compact"])
+def test_256cube_fp64_steps_bit_exact(oracle256, lattice256, mode):
+    import ctypes as C
+    from flowperm import _lib, solver
+    ctx = solver.Context()
+    try:
+        t = _lib.Tuning()
+        ctx.lib.fp_get_tuning(ctx.handle, C.byref(t))
+        t.nbr_mode = mode
+        assert ctx.lib.fp_set_tuning(ctx.handle, C.byref(t)) == 0
+        print(f"256^3 FP64 6 steps, form {mode}, porosity {1 - lattice256.mean():.4f}", flush=True)
+        g = solver.solve_direction3d(lattice256, F6, ctx=ctx, velocity=True, precision="f64", **STEPS6)
+    finally:
+        ctx.close()
+    o = oracle256
     assert g["steps"] == o["steps"] == 6
     assert np.array_equal(g["q"], o["q"])
     assert np.array_equal(g["velocity"], o["velocity"])
