@@ -62,6 +62,24 @@ def test_256cube_fp64_steps_bit_exact(oracle256, lattice256, mode):
     assert np.all(g["velocity"][lattice256 == 1] == 0)
 
 
+def test_4gib_fp64_wide_addressing_steps_bit_exact(gpu_ctx, oracle):
+    """A lattice whose FP64 populations span more than 4 GiB: the neighbour
+    kernels switch to 64-bit addresses by themselves (28.9 M fluid cells,
+    3% isolated solid voxels, row-segment form, y bands)."""
+    from flowperm import solver
+    rng = np.random.default_rng(11)
+    m = (rng.random((300, 310, 320)) < 0.03).astype(np.uint8)
+    n_fluid = int((m == 0).sum())
+    stride = (n_fluid + 1023) // 1024 * 1024
+    assert 19 * 8 * stride > 2 ** 32
+    cfg = dict(ff=40, lag=2, min_steps=10 ** 6, max_steps=4, avg_steps=0)
+    print(f"300x310x320 FP64 4 steps, {n_fluid} fluid cells, {19 * 8 * stride / 2 ** 30:.2f} GiB", flush=True)
+    g = solver.solve_direction3d(m, F6, ctx=gpu_ctx, precision="f64", **cfg)
+    o = oracle.solve_dir3d(m, F6, **cfg)
+    assert g["steps"] == o["steps"] == 4
+    assert np.array_equal(g["q"], o["q"])
+
+
 def test_256cube_fp32_alone_equals_batched_and_fp64(gpu_ctx, lattice256):
     from flowperm import masks, solver
     from flowperm.server import Server
