@@ -554,7 +554,7 @@ std::string Batch::rebuild()
         // large lattices deal their neighbour-step items to the XCDs in y bands
         // (see below)
         const bool banded = g.dim == 3 &&
-            (double)lattice_q(lattice_) * (double)j->stride * (double)elem_bytes(prec_) >= 4e8;
+            (double)lattice_q(lattice_) * (double)j->stride * (double)elem_bytes(prec_) >= BAND_MIN_BYTES;
         if(j->nbr_mode == 1) {
             // compact neighbour step: up to 256 consecutive fluid cells per
             // item; its partial slots follow the row-segment ones (offset

@@ -538,6 +538,12 @@ struct Tuning {
 // at 0.62, 1.02 / 1.04 at 0.72, 1.05 / 1.06 at 0.92.
 constexpr double COMPACT_BELOW_POROSITY = 0.68;
 
+// From this many bytes of populations a 3D instance deals its neighbour-step
+// items to the 8 XCDs in y bands through all z instead of z ranges (the slabs
+// either side of a boundary then share an L2).  Below it the z ranges stay, so
+// that an XCD's neighbour step and local step cover the same cells.
+constexpr double BAND_MIN_BYTES = 4e8;
+
 // work of one workgroup of the local (in-place collide) step: a run of compact cells
 struct LocalWork {
     int32_t inst;
