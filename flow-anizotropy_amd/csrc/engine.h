@@ -64,6 +64,7 @@ struct Job {
     float *d_vel[2] = {nullptr, nullptr};
     bool want_fields = false;        // keep the velocity fields (3D export, 2D fieldres)
     int nbr_mode = 0;                // neighbour-step form in use (set by the batch)
+    int64_t items = 0;               // its work items in the batch's current tables
     InstState state{};
     bool admitted = false, done = false;
     int64_t id = -1;

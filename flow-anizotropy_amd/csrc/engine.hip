@@ -620,11 +620,13 @@ std::string Batch::rebuild()
         }
         desc[i].np1 = (int32_t)nw.size();
     }
-    for(int i = 0; i < n_inst_; i++)
+    for(int i = 0; i < n_inst_; i++) {
         if(jobs_[i]->nbr_mode == 1) {
             desc[i].np0 += (int32_t)nw.size();
             desc[i].np1 += (int32_t)nw.size();
         }
+        jobs_[i]->items = (int64_t)(desc[i].lp1 - desc[i].lp0) + (desc[i].np1 - desc[i].np0);
+    }
     // second-level reduction items over each instance's partial ranges
     std::vector<RedWork> lr, nr;
     for(int i = 0; i < n_inst_; i++) {
@@ -758,13 +760,7 @@ std::string Batch::run_chunk()
 
 int64_t Batch::work_items(const Job &j) const
 {
-    const Geometry &g = *j.geo;
-    const int64_t lchunk = 256 * (int64_t)tune_.local_cpt;
-    const int64_t planes = g.dim == 3 ? g.nz : g.ny;
-    const int64_t ustride = (g.dim == 3 ? g.ny : 1) * (int64_t)g.nseg;
-    const int64_t nbr = j.nbr_mode == 1 ? (g.n_fluid + 255) / 256
-                                        : (planes + tune_.nbr_waves - 1) / tune_.nbr_waves * ustride;
-    return (g.n_fluid + lchunk - 1) / lchunk + nbr;
+    return j.items;
 }
 
 void Batch::launch_finish(bool local)
