@@ -432,6 +432,55 @@ int fp_verify_links(const uint8_t *solid, int dim, int nx, int ny, int nz, int64
     return e.empty() ? FP_OK : status_of_geometry_error(e);
 }
 
+// ---------------------------------------------------------------------------
+// host-only checks of the convergence rules and the field residual
+// ---------------------------------------------------------------------------
+// The device state machine (advance_state / plan_next, the code k_finish runs
+// after every step) driven by a given flux sequence: step t (1-based) sees
+// q[(t - 1) * dim ...] when the machine asked that step for a flux, and a zero
+// flux otherwise, as on the device.
+static int replay_convergence(const InstCfg &c, int dim, const double *q, int64_t n, fp_solve_out *out)
+{
+    InstState S;
+    init_state(c, S);
+    for(int64_t t = 0; S.phase < PH_DONE; t++) {
+        if(t >= n) return FP_ERR_ARG;
+        const int flux = S.flux_next;
+        double qt[3] = {0.0, 0.0, 0.0};
+        if(flux)
+            for(int k = 0; k < dim; k++) qt[k] = q[t * dim + k];
+        advance_state(c, S, qt, flux);
+        plan_next(c, S);
+    }
+    Job j;
+    j.state = S;
+    fill_out(j, out);
+    if(S.status == 2) return FP_ERR_NONFINITE;
+    return S.converged ? FP_OK : FP_NOT_CONVERGED;
+}
+
+int fp_replay_convergence3d(const fp_config3d *cfg, const double *q, int64_t nsteps, fp_solve_out *out)
+{
+    if(!cfg3_ok(cfg) || !q || !out || nsteps < 0) return FP_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    return replay_convergence(cfg3(*cfg, 1, false), 3, q, nsteps, out);
+}
+
+int fp_replay_convergence2d(const fp_config2d *cfg, const double *q, int64_t nsteps, fp_solve_out *out)
+{
+    if(!cfg2_ok(cfg) || !q || !out || nsteps < 0) return FP_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    return replay_convergence(cfg2(*cfg, 1, 1), 2, q, nsteps, out);
+}
+
+double fp_field_residual2d(int lx, int ly, const float *cur, const float *snap)
+{
+    if(lx <= 0 || ly <= 0 || !cur || !snap) return NAN;
+    return field_residual(lx, ly, cur, snap);
+}
+
+void fp_test_fail_rebuilds(int n) { test_fail_rebuilds(n); }
+
 int fp_check_mask3d(const uint8_t *solid, int nx, int ny, int nz, int64_t *n_pore)
 {
     if(nx <= 0 || ny <= 0 || nz <= 0) return FP_ERR_BAD_DIMS;

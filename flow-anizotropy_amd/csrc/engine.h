@@ -180,4 +180,8 @@ void f32_force_terms(const double F[3], float g45[19], float h15[19]);
 
 double now_seconds();
 
+// Make the next n table rebuilds of any batch in the process fail as a device
+// error would, before they touch the device (tests of the error paths).
+void test_fail_rebuilds(int n);
+
 } // namespace fp

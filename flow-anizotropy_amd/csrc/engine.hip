@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -483,6 +484,10 @@ std::string Batch::admit(Job *j)
     return "";
 }
 
+static std::atomic<int> g_fail_rebuilds{0};
+
+void test_fail_rebuilds(int n) { g_fail_rebuilds.store(n < 0 ? 0 : n); }
+
 template<class T>
 static hipError_t ensure(T *&p, size_t &cap, size_t n)
 {
@@ -497,6 +502,8 @@ static hipError_t ensure(T *&p, size_t &cap, size_t n)
 
 std::string Batch::rebuild()
 {
+    for(int left = g_fail_rebuilds.load(); left > 0; left = g_fail_rebuilds.load())
+        if(g_fail_rebuilds.compare_exchange_weak(left, left - 1)) return "injected device failure";
     hipError_t e = hipStreamSynchronize(stream_);
     if(e != hipSuccess) return hip_err("stream", e);
     release_graph();

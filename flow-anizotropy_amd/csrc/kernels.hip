@@ -199,7 +199,9 @@ __device__ __forceinline__ void write_vel(const InstDesc &D, int slot, int64_t i
 }
 
 // ---------------------------------------------------------------------------
-// rest state: f = w (unshifted) or h = 0 (shifted)
+// rest state: f = w (unshifted) or h = 0 (shifted), and zero velocity fields
+// (resetdistributions sets U = V = 0, LMB2d/lbm.cpp:200-211: a 2D solve
+// whose first phase runs no step snapshots that rest field)
 // ---------------------------------------------------------------------------
 template<class L, int P>
 __global__ void __launch_bounds__(256)
@@ -217,6 +219,15 @@ k_rest(InstDesc D)
         else if constexpr(P == PREC_F64) w = (T)(k == 0 ? 4.0/9.0 : (k < 5 ? 1.0/9.0 : 1.0/36.0));
         else w = (T)L::wf(k);
         f[pop_at<L::Q>(k, (uint32_t)i)] = w;
+    }
+    if(i < D.n_fluid) {
+#pragma unroll
+        for(int s = 0; s < 2; s++) {
+            if(!D.vel[s]) continue;
+            gptr<float> v = as_global<float>(D.vel[s]);
+#pragma unroll
+            for(int c = 0; c < L::DIM; c++) v[i * L::DIM + c] = 0.0f;
+        }
     }
 }
 

@@ -389,7 +389,7 @@ FP_HD void fail_state(InstState &S)
 
 FP_HD bool all_finite(const double *q, int nd)
 {
-    for(int k = 0; k < nd; k++) if(!isfinite(q[k])) return false;
+    for(int k = 0; k < nd; k++) if(!__builtin_isfinite(q[k])) return false;
     return true;
 }
 

@@ -27,6 +27,7 @@ struct Request {
     std::vector<std::unique_ptr<Job>> jobs;        // forcing runs (+ the validate run, last)
     std::vector<fp_solve_out> outs;
     int nruns = 0, ndone = 0;
+    int bypassed = 0;                              // later requests admitted ahead of it
     bool validate = false;
     size_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -41,10 +42,11 @@ struct Request {
 struct fp_server {
     fp_ctx *ctx = nullptr;
     int lattice = 3, prec = FP_F32, chunk = 200;
+    int max_bypass = 16;
     int64_t budget = 0;
     std::unique_ptr<Batch> batch;                  // touched by the stepping thread only
     std::mutex mu;                                 // guards everything below
-    std::deque<Request *> queue;                   // FIFO of requests waiting for memory
+    std::deque<Request *> queue;                   // requests waiting for memory, oldest first
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;   // every unpolled request
     std::unordered_map<const Job *, Request *> owner;
     int64_t next_id = 0;
@@ -144,17 +146,27 @@ int finish_request(Request *r)
     return status;
 }
 
-// admit queued requests in arrival order while they fit under the budget
+// Admit queued requests that fit under the budget.  The queue is walked in
+// arrival order; a request that does not fit is passed over and later ones that
+// do fit are admitted ahead of it (a small request does not wait behind a large
+// one for memory it does not need), until it has been passed over max_bypass
+// times: from then on nothing behind it is admitted before it.  max_bypass < 0
+// keeps strict arrival order.
 void admit_queued(fp_server *S)
 {
     for(;;) {
         Request *r = nullptr;
+        size_t pos = 0;
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            if(S->queue.empty()) return;
-            r = S->queue.front();
-            if(S->in_use + (int64_t)r->bytes > S->budget) return;
-            S->queue.pop_front();
+            for(size_t i = 0; i < S->queue.size(); i++) {
+                Request *q = S->queue[i];
+                if(S->in_use + (int64_t)q->bytes <= S->budget) { r = q; pos = i; break; }
+                if(S->max_bypass < 0 || q->bypassed >= S->max_bypass) break;
+            }
+            if(!r) return;
+            for(size_t i = 0; i < pos; i++) S->queue[i]->bypassed++;
+            S->queue.erase(S->queue.begin() + (std::ptrdiff_t)pos);
             S->in_use += (int64_t)r->bytes;
             S->peak = std::max(S->peak, S->in_use);
         }
@@ -181,7 +193,9 @@ void admit_queued(fp_server *S)
             S->st.rejected++;
             continue;
         }
-        S->queue.push_front(r);
+        // back to its place (other threads only append to the queue)
+        for(size_t i = 0; i < pos; i++) S->queue[i]->bypassed--;
+        S->queue.insert(S->queue.begin() + (std::ptrdiff_t)pos, r);
         return;
     }
 }
@@ -219,6 +233,37 @@ void collect(fp_server *S)
     }
 }
 
+// A failed chunk (a device error, or a table rebuild that could not get its
+// memory) ends every resident request: its runs leave the batch, its lattices
+// and footprint are returned, and it finishes with FP_ERR_DEVICE.  Nothing of
+// it stays behind for a later step or poll to touch; queued requests are
+// untouched and are admitted by later steps.
+void fail_resident(fp_server *S, const std::string &err)
+{
+    std::vector<Request *> failed;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->ctx->err = err;
+        for(auto &kv : S->reqs) {
+            Request *r = kv.second.get();
+            if(r->status == FP_PENDING && r->t_admit > 0) failed.push_back(r);
+        }
+    }
+    for(Request *r : failed)
+        for(auto &j : r->jobs)
+            if(j->admitted) S->batch->release(j.get());
+    std::lock_guard<std::mutex> lk(S->mu);
+    const double t = now_seconds();
+    for(Request *r : failed) {
+        for(auto &j : r->jobs) S->owner.erase(j.get());
+        r->status = FP_ERR_DEVICE;
+        r->t_done = t;
+        r->geo.reset();
+        S->in_use -= (int64_t)r->bytes;
+        S->st.completed++;
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -241,6 +286,7 @@ fp_server *fp_server_create(fp_ctx *ctx, const fp_server_config *cfg, int *statu
     S->lattice = L;
     S->prec = P;
     S->chunk = cfg->chunk_steps > 0 ? cfg->chunk_steps : ctx->chunk;
+    S->max_bypass = cfg->max_bypass == 0 ? 16 : (cfg->max_bypass < 0 ? -1 : cfg->max_bypass);
     if(cfg->budget_bytes > 0) S->budget = cfg->budget_bytes;
     else {
         size_t fr = 0, tot = 0;
@@ -310,16 +356,7 @@ int fp_server_step(fp_server *S, int64_t *pending)
         std::string e = S->batch->run_chunk();
         const double dt = now_seconds() - t0;
         if(!e.empty()) {
-            // a device failure ends every resident request
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->ctx->err = e;
-            for(auto &kv : S->reqs) {
-                Request *r = kv.second.get();
-                if(r->status == FP_PENDING && r->t_admit > 0) {
-                    r->status = FP_ERR_DEVICE;
-                    r->t_done = now_seconds();
-                }
-            }
+            fail_resident(S, e);
             rc = FP_ERR_DEVICE;
         } else {
             collect(S);

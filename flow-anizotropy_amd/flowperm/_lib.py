@@ -97,7 +97,7 @@ class Tuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("lattice", C.c_int32), ("precision", C.c_int32), ("budget_bytes", C.c_int64),
-                ("chunk_steps", C.c_int32), ("pad", C.c_int32)]
+                ("chunk_steps", C.c_int32), ("max_bypass", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -168,6 +168,10 @@ SIGNATURES = {
     "fp_server_poll2d": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result2D),
                                    C.POINTER(RequestInfo)]),
     "fp_server_get_stats": (None, [C.c_void_p, C.POINTER(ServerStats)]),
+    "fp_replay_convergence3d": (C.c_int, [C.POINTER(Config3D), _dp, C.c_int64, C.POINTER(SolveOut)]),
+    "fp_replay_convergence2d": (C.c_int, [C.POINTER(Config2D), _dp, C.c_int64, C.POINTER(SolveOut)]),
+    "fp_field_residual2d": (C.c_double, [C.c_int, C.c_int, _fp, _fp]),
+    "fp_test_fail_rebuilds": (None, [C.c_int]),
 }
 
 _lib = None

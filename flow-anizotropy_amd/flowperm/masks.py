@@ -15,7 +15,9 @@ periodic boundaries.
                    counterpart.
 * ``grf``          Gaussian random field (FFT-filtered white noise with
                    anisotropic Gaussian correlation lengths), thresholded to a
-                   porosity and reduced to its largest percolating pore cluster.
+                   porosity and reduced to its largest percolating pore cluster
+                   (optionally with the threshold corrected so that the kept
+                   cluster has the target porosity).
 * ``percolates``   for each axis, whether some pore cluster wraps around the
                    torus along it (6-connectivity in 3D, 4 in 2D).  This is what
                    scripts/gen_aniso_structures3d.py:184-201 tests by labelling
@@ -28,7 +30,7 @@ from __future__ import annotations
 import numpy as np
 from scipy import ndimage
 
-__all__ = ["spheres", "ellipsoids", "ellipses", "grf", "percolates",
+__all__ = ["spheres", "ellipsoids", "ellipses", "grf", "grf_field", "percolates",
            "percolating_cluster", "slit", "random_unit_vector"]
 
 
@@ -217,9 +219,9 @@ def percolating_cluster(solid, require_all=False):
     return (root_lab != best).astype(np.uint8)
 
 
-def grf(shape=(128, 128, 128), porosity=0.2, corr=(8.0, 8.0, 24.0), seed=0,
-        keep_percolating=True, require_all=False):
-    """Gaussian random field mask.  corr is given per solver axis (x, y, z)."""
+def grf_field(shape=(128, 128, 128), corr=(8.0, 8.0, 24.0), seed=0):
+    """FFT-filtered white noise with anisotropic Gaussian correlation lengths
+    corr, given per solver axis (x, y, z); float32, array order."""
     rng = np.random.default_rng(seed)
     shape = tuple(int(s) for s in shape)
     dim = len(shape)
@@ -237,12 +239,45 @@ def grf(shape=(128, 128, 128), porosity=0.2, corr=(8.0, 8.0, 24.0), seed=0,
         filt = g if filt is None else filt * g
     spec *= filt
     field = np.fft.irfftn(spec, s=shape, axes=list(range(dim))).astype(np.float32)
-    del spec
-    t = np.quantile(field, porosity)
-    solid = (field > t).astype(np.uint8)              # pore fraction = porosity
-    del field
-    if keep_percolating:
-        kept = percolating_cluster(solid, require_all=require_all)
-        if kept is not None:
-            solid = kept
-    return solid
+    return field
+
+
+def grf(shape=(128, 128, 128), porosity=0.2, corr=(8.0, 8.0, 24.0), seed=0,
+        keep_percolating=True, require_all=False, cluster_porosity=False, tol=0.002):
+    """Gaussian random field mask: pore where the field is below the quantile
+    `porosity`, then (keep_percolating) only the largest percolating pore
+    cluster.  The isolated pockets that reduction removes lower the porosity;
+    with cluster_porosity the threshold is raised until the kept cluster itself
+    has porosity within tol of the target (a few passes of
+    threshold += target - achieved)."""
+    field = grf_field(shape, corr, seed)
+    flat = field.ravel()
+
+    def threshold(p):
+        return (field > np.quantile(flat, min(max(p, 0.0), 1.0))).astype(np.uint8)
+
+    solid = threshold(porosity)
+    if not keep_percolating:
+        return solid
+    kept = percolating_cluster(solid, require_all=require_all)
+    if not cluster_porosity:
+        return kept if kept is not None else solid
+    # the kept cluster's porosity grows with the threshold level p: secant
+    # steps p += target - achieved, bisecting once the target is bracketed
+    best, p, lo, hi = None, porosity, None, None
+    for _ in range(16):
+        got = 1.0 - float(kept.mean()) if kept is not None else 0.0
+        if kept is not None and (best is None or abs(got - porosity) < best[0]):
+            best = (abs(got - porosity), kept)
+            if best[0] <= tol:
+                break
+        if got < porosity:
+            lo = p if lo is None else max(lo, p)
+        else:
+            hi = p if hi is None else min(hi, p)
+        nxt = p + (porosity - got) if kept is not None else p + 0.02
+        if lo is not None and hi is not None and not lo < nxt < hi:
+            nxt = 0.5 * (lo + hi)
+        p = nxt
+        kept = percolating_cluster(threshold(p), require_all=require_all)
+    return best[1] if best is not None else solid

@@ -28,14 +28,14 @@ from .solver import Context, _mask2d, _mask3d, _precision, config2d, config3d, r
 
 class Server:
     def __init__(self, lattice: int = 3, precision="f32", budget_bytes: int = 0,
-                 chunk_steps: int = 0, ctx: Context | None = None):
+                 chunk_steps: int = 0, ctx: Context | None = None, max_bypass: int = 0):
         self.ctx = ctx or Context()
         self._own_ctx = ctx is None
         self.lib = self.ctx.lib
         self.lattice = int(lattice)
         self.precision = precision
         cfg = _lib.ServerConfig(self.lattice, _precision(precision), int(budget_bytes),
-                                int(chunk_steps), 0)
+                                int(chunk_steps), int(max_bypass))
         st = C.c_int(0)
         self.handle = self.lib.fp_server_create(self.ctx.handle, C.byref(cfg), C.byref(st))
         if not self.handle:

@@ -11,19 +11,20 @@ bad dimensions, truncated payload, trailing bytes, non-binary voxels and a
 mask with no pore space are all errors (``StructureError``).
 
 2D inputs follow LMB2d/lbm.cpp:412-483: a GIF pixel is pore when its red
-channel is > 0 (``read_from_gif``), a PPM cell is pore when its R value is
-non-zero, a DAT cell is pore when it is 0.  Arrays are indexed ``[y, x]``
+channel is > 0 (``read_from_gif``), a PPM cell is pore when atoi of its R token
+is non-zero, a DAT cell is pore when atoi of its token is 0.  Arrays are indexed ``[y, x]``
 (row = solver y, column = solver x, scripts/structure_io.py:44-54).  The GIF
 decode uses PIL instead of Magick++.
 """
 from __future__ import annotations
 
+import re
 from pathlib import Path
 
 import numpy as np
 
 __all__ = ["StructureError", "save_raw", "load_raw", "check_mask3d", "porosity",
-           "load_structure2d", "save_gif2d"]
+           "load_structure2d", "save_gif2d", "atoi"]
 
 HEADER_BYTES = 12
 
@@ -89,15 +90,41 @@ def porosity(solid) -> float:
     return float(1.0 - np.asarray(solid).mean())
 
 
-def load_structure2d(path, shape=None) -> np.ndarray:
-    """Read a 2D structure as uint8 [y, x], 1 = solid (lbm.cpp:412-491 rules).
+_ATOI = re.compile(r"[+-]?[0-9]+")
 
-    GIF: pore where red > 0.  PPM (P3): four header lines are skipped as the
-    reference does, then LY rows of LX (R, G, B) triples; pore where R != 0.
-    DAT: LY rows of LX integers, pore where the value is 0.  The reference
-    reads exactly LX*LY values of its compiled 256x256 lattice; here the PPM
-    shape comes from its header and a DAT file is taken as square unless
-    ``shape=(ly, lx)`` is given.
+
+def atoi(token: str) -> int:
+    """C atoi of one token as the reference's ``>>`` extraction yields it (no
+    whitespace): an optional sign and the leading digits, 0 when there are none
+    ("12abc" -> 12, "abc" -> 0, "0.5" -> 0, "-3" -> -3)."""
+    m = _ATOI.match(token)
+    return int(m.group(0)) if m else 0
+
+
+def _ppm_dims(lines):
+    """(lx, ly) from the first of header lines 2-3 that holds two integers"""
+    for line in lines:
+        tok = line.split()
+        if len(tok) >= 2 and _ATOI.fullmatch(tok[0]) and _ATOI.fullmatch(tok[1]):
+            return int(tok[0]), int(tok[1])
+    return None
+
+
+def load_structure2d(path, shape=None) -> np.ndarray:
+    """Read a 2D structure as uint8 [y, x], 1 = solid (LMB2d/lbm.cpp:412-491).
+
+    GIF (read_from_gif, :412-426): pore where the pixel's red channel is > 0.
+    PPM (read_from_ppm, :427-454): four header lines are skipped, then LY rows
+    of LX (R, G, B) tokens; pore where atoi(R) != 0.  DAT (read_from_dat,
+    :456-470): LY rows of LX tokens, pore where atoi(token) == 0.  The file
+    extension picks the reader (:472-483: ".ppm", ".gif", anything else DAT).
+
+    The reference reads exactly LX*LY values of its compiled 256x256 lattice;
+    here the PPM shape comes from its header (line 2, or line 3 when line 2 is
+    a comment) and a DAT file is taken as square unless ``shape=(ly, lx)`` is
+    given.  A file with fewer values than the lattice needs is an error here
+    (the reference would read the missing tokens as empty strings: solid in a
+    PPM, pore in a DAT).
     """
     path = str(path)
     if path.endswith(".gif"):
@@ -108,28 +135,39 @@ def load_structure2d(path, shape=None) -> np.ndarray:
         except OSError as e:
             raise StructureError(f"cannot read {path}") from e
         return (red == 0).astype(np.uint8)
-    if path.endswith(".ppm"):
+    try:
         with open(path) as fh:
-            fh.readline()
-            dims = fh.readline().split()
-            fh.readline()
-            fh.readline()
-            vals = fh.read().split()
-        lx, ly = (int(dims[0]), int(dims[1])) if shape is None else (shape[1], shape[0])
-        if len(vals) < 3 * lx * ly:
+            if path.endswith(".ppm"):
+                head = [fh.readline() for _ in range(4)]
+                toks = fh.read().split()
+            else:
+                head = None
+                toks = fh.read().split()
+    except OSError as e:
+        raise StructureError(f"cannot open {path}") from e
+    if head is not None:
+        if shape is None:
+            dims = _ppm_dims(head[1:3])
+            if dims is None:
+                raise StructureError(f"{path}: no dimensions in the PPM header")
+            lx, ly = dims
+        else:
+            ly, lx = shape
+        if lx <= 0 or ly <= 0:
+            raise StructureError(f"bad dimensions in {path}")
+        if len(toks) < 3 * lx * ly:
             raise StructureError(f"truncated payload in {path}")
-        r = np.array([int(v) for v in vals[0:3 * lx * ly:3]], dtype=np.int64)
-        return (r == 0).astype(np.uint8).reshape(ly, lx)
-    with open(path) as fh:
-        vals = np.array([int(v) for v in fh.read().split()], dtype=np.int64)
+        red = np.array([atoi(t) for t in toks[0:3 * lx * ly:3]], dtype=np.int64)
+        return (red == 0).astype(np.uint8).reshape(ly, lx)
     if shape is None:
-        n = int(round(np.sqrt(vals.size)))
-        if n * n != vals.size:
+        n = int(round(np.sqrt(len(toks))))
+        if n * n != len(toks):
             raise StructureError(f"{path}: DAT payload is not square; pass shape=(ly, lx)")
         shape = (n, n)
-    if vals.size < shape[0] * shape[1]:
+    if len(toks) < shape[0] * shape[1]:
         raise StructureError(f"truncated payload in {path}")
-    return (vals[:shape[0] * shape[1]] != 0).astype(np.uint8).reshape(shape)
+    vals = np.array([atoi(t) for t in toks[:shape[0] * shape[1]]], dtype=np.int64)
+    return (vals != 0).astype(np.uint8).reshape(shape)
 
 
 def save_gif2d(path, solid) -> Path:

@@ -6,9 +6,11 @@ Seeded structure sets of the solver's measured workloads.
   seeded random axis with 15-degree jitter, porosity ~ U(0.3, 0.8).
 * ``serving_structure(i, seed)``: the serving set: cube edge log-uniform in
   [32, 256], spheres (radius U[3, 6]) or a Gaussian random field (correlation
-  lengths 8, 8, 24 voxels) with equal odds, porosity ~ U(0.2, 0.8).
-* ``large_grf(seed)``: the 512^3 low-porosity Gaussian-random-field lattice
-  (porosity ~ U(0.15, 0.25), reduced to its largest cluster that percolates).
+  lengths 8, 8, 24 voxels, its percolating cluster at the target porosity) with
+  equal odds, porosity ~ U(0.2, 0.8).
+* ``large_grf(seed)``: the 512^3 low-porosity Gaussian-random-field lattice:
+  its largest cluster that percolates, with porosity ~ U(0.15, 0.25) (the
+  threshold is corrected for the pockets the cluster reduction removes).
 
 Every mask is kept only if its pore space wraps the torus along x, y and z
 (the test of scripts/gen_aniso_structures3d.py:184-201): a direction without a
@@ -52,7 +54,7 @@ def serving_structure(i: int, seed: int = 0) -> np.ndarray:
             m = cl if cl is not None else m
         else:
             m = masks.grf((edge,) * 3, porosity=por, corr=(8.0, 8.0, 24.0), seed=sub,
-                          require_all=True)
+                          require_all=True, cluster_porosity=True)
         if _percolating(m) and (m == 0).any():
             return m
     raise RuntimeError(f"no percolating serving structure for index {i}")
@@ -63,8 +65,8 @@ def large_grf(seed: int = 0, n: int = 512) -> np.ndarray:
         rng = np.random.default_rng([seed, 7, attempt])
         por = float(rng.uniform(0.15, 0.25))
         m = masks.grf((n, n, n), porosity=por, corr=(8.0, 8.0, 24.0),
-                      seed=int(rng.integers(1 << 31)), require_all=True)
-        if _percolating(m):
+                      seed=int(rng.integers(1 << 31)), require_all=True, cluster_porosity=True)
+        if _percolating(m) and 0.15 <= 1.0 - float(m.mean()) <= 0.25:
             return m
     raise RuntimeError("no percolating large GRF lattice")
 

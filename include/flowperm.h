@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 5
+#define FP_ABI_VERSION 6
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -178,6 +178,26 @@ typedef struct fp_config2d {
 
 void fp_config2d_default(fp_config2d *cfg);
 
+/* field_residual of LMB2d/lbm.cpp:169-182 (the fieldres_* columns): the L2
+ * change between two velocity fields, both [ly][lx] rows of (U, V) floats,
+ * relative to the first: sqrt(sum |cur - snap|^2 / sum |cur|^2), rounded to
+ * float, or 10 when cur is zero everywhere.  Host only. */
+double fp_field_residual2d(int lx, int ly, const float *cur, const float *snap);
+
+/* The convergence rules of solve() alone, without a lattice (host only, no
+ * device work): the state machine the device advances after every step,
+ * driven by a given flux sequence.  q holds nsteps x 3 (2D: x 2) values; step
+ * t (1-based) sees q[(t-1)*3 ..] as the flux volumeavg would return after it
+ * when the rules ask for one that step, and nothing otherwise.  Fills out
+ * (q, se, steps, converged, status) as the solve would and returns FP_OK,
+ * FP_NOT_CONVERGED or FP_ERR_NONFINITE; FP_ERR_ARG when the sequence ends
+ * before the solve does.  3D: phase 1 and 2 of LMB3d/lbm3d-perm.cpp:320-398;
+ * 2D: the block-mean / k-sigma test and averaging of LMB2d/main.cpp:99-235. */
+int fp_replay_convergence3d(const fp_config3d *cfg, const double *q, int64_t nsteps,
+                            fp_solve_out *out);
+int fp_replay_convergence2d(const fp_config2d *cfg, const double *q, int64_t nsteps,
+                            fp_solve_out *out);
+
 /* Everything lbm2d-perm writes to its CSV row (LMB2d/main.cpp:382-411). */
 typedef struct fp_result2d {
     int32_t lx, ly;
@@ -304,9 +324,11 @@ void fp_tune_shape(int kind, int i, int *a, int *b, int *c);
  *
  * A request is one whole lbm3d-perm / lbm2d-perm invocation: its 3 (2D: 2)
  * forcing runs, plus the --validate run when asked, become instances of one
- * continuously batched stepper.  Requests wait in a FIFO until the device
- * memory they need (segment table + population arrays) fits under the budget;
- * admitted requests join the running batch at the next chunk boundary, each
+ * continuously batched stepper.  Requests wait until the device memory they
+ * need (segment table + population arrays) fits under the budget, oldest
+ * first; a request that fits may be admitted ahead of an older one that does
+ * not, at most max_bypass times per waiting request.  Admitted requests join
+ * the running batch at the next chunk boundary, each
  * instance retires on the device when its own convergence test passes, and a
  * request's memory is returned as soon as its last instance retires.
  * A request's result does not depend on what else is resident (every instance
@@ -323,7 +345,10 @@ typedef struct fp_server_config {
     int64_t budget_bytes;     /* device memory for resident requests; 0 = 90% of
                                  the free memory when the server is created      */
     int32_t chunk_steps;      /* steps between admissions (even); 0 = context's  */
-    int32_t pad;
+    int32_t max_bypass;       /* later requests that may be admitted ahead of a
+                                 waiting one that does not fit yet, before it
+                                 blocks the queue; 0 = 16, < 0 = strict arrival
+                                 order                                          */
 } fp_server_config;
 
 typedef struct fp_request_info {
@@ -367,6 +392,11 @@ int        fp_server_step(fp_server *s, int64_t *pending);
 int        fp_server_poll3d(fp_server *s, int64_t id, fp_result3d *out, fp_request_info *info);
 int        fp_server_poll2d(fp_server *s, int64_t id, fp_result2d *out, fp_request_info *info);
 void       fp_server_get_stats(fp_server *s, fp_server_stats *out);
+
+/* Tests of the error paths (no reference counterpart): the next n table
+ * rebuilds of any batch or server in the process fail as a device error
+ * would, before they touch the device.  0 clears it. */
+void fp_test_fail_rebuilds(int n);
 
 /* Plain device-to-device copy of `bytes` (read + write), `iters` times:
  * the measured bandwidth the step kernels are compared with. */

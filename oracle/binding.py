@@ -95,6 +95,11 @@ def lib():
             getattr(_lib, pfx + "run_steps").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double,
                                                          C.c_double, C.c_long, C.c_void_p, C.c_void_p]
             getattr(_lib, pfx + "mu").restype = C.c_float
+            getattr(_lib, pfx + "replay").argtypes = [C.POINTER(Cfg2), C.c_void_p, C.c_long,
+                                                      C.POINTER(Solve2)]
+            getattr(_lib, pfx + "field_residual").argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+            getattr(_lib, pfx + "field_residual").restype = C.c_double
+        _lib.or3_replay.argtypes = [C.POINTER(Cfg3), C.c_void_p, C.c_long, C.POINTER(Solve3)]
     return _lib
 
 
@@ -216,3 +221,33 @@ def run_reference(raw_path, csv_path, flags=(), cwd=None, timeout=None):
         raise RuntimeError(f"{REF_BIN} missing: run `make -C oracle ref` where the reference is mounted")
     return subprocess.run([str(REF_BIN), str(raw_path), str(csv_path)] + list(flags),
                           capture_output=True, text=True, cwd=cwd, timeout=timeout)
+
+
+def replay3d(seq, **cfg):
+    """solve()'s convergence rules alone over a planted flux sequence
+    (seq[s-1] = the flux after step s, shape (n, 3))."""
+    q = np.ascontiguousarray(seq, dtype=np.float64).reshape(-1, 3)
+    out = Solve3()
+    lib().or3_replay(C.byref(cfg3(**cfg)), q.ctypes.data_as(C.c_void_p), q.shape[0], C.byref(out))
+    return {"q": np.array(out.q[:]), "se": np.array(out.se[:]), "steps": out.steps,
+            "converged": out.converged, "status": out.status}
+
+
+def replay2d(seq, double=False, **cfg):
+    """The 2D rules (block means, k-sigma, averaging) over a planted sequence
+    of shape (n, 2)."""
+    q = np.ascontiguousarray(seq, dtype=np.float64).reshape(-1, 2)
+    out = Solve2()
+    getattr(lib(), _p2(double) + "replay")(C.byref(cfg2(**cfg)), q.ctypes.data_as(C.c_void_p),
+                                           q.shape[0], C.byref(out))
+    return {"q": np.array([out.qx, out.qy]), "se": np.array([out.se_qx, out.se_qy]),
+            "steps": out.steps, "converged": out.converged, "status": out.status}
+
+
+def field_residual2d(cur, snap, double=False):
+    """field_residual of two [y][x] (U, V) float fields."""
+    c = np.ascontiguousarray(cur, dtype=np.float32)
+    sn = np.ascontiguousarray(snap, dtype=np.float32)
+    ly, lx = c.shape[:2]
+    return getattr(lib(), _p2(double) + "field_residual")(lx, ly, c.ctypes.data_as(C.c_void_p),
+                                                          sn.ctypes.data_as(C.c_void_p))

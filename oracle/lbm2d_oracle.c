@@ -270,13 +270,27 @@ static void lat2_free(lat2 *L)
 
 static inline double dmax2(double a, double b) { return (a < b) ? b : a; }
 
-/* main.cpp:99-235 */
-static void solve2(lat2 *L, const or2_cfg *cfg, double Fx, double Fy, or2_solve *out, float *vel)
+/* The flux solve() reads after step s: volumeavg of the lattice, or, with no
+ * lattice (L == NULL), entry s of a planted sequence (seq[(s-1)*2 + c]; NaN past
+ * its end), which tests use to drive the convergence rules alone. */
+static void flux2(const lat2 *L, const double *seq, long nseq, long s, double *qx, double *qy)
+{
+    if(L) { volumeavg2(L, qx, qy); return; }
+    *qx = (s >= 1 && s <= nseq) ? seq[(s - 1) * 2] : NAN;
+    *qy = (s >= 1 && s <= nseq) ? seq[(s - 1) * 2 + 1] : NAN;
+}
+
+/* main.cpp:99-235.  With L == NULL nothing is stepped and the flux comes from
+ * seq (flux2); fieldres is then 0. */
+static void solve2(lat2 *L, const or2_cfg *cfg, double Fx, double Fy, or2_solve *out, float *vel,
+                   const double *seq, long nseq)
 {
     memset(out, 0, sizeof(*out));
-    reset2(L);
-    L->fx = FORCE(Fx);
-    L->fy = FORCE(Fy);
+    if(L) {
+        reset2(L);
+        L->fx = FORCE(Fx);
+        L->fy = FORCE(Fy);
+    }
 
     long s = 0;
     int  hold = 0;
@@ -287,13 +301,13 @@ static void solve2(lat2 *L, const or2_cfg *cfg, double Fx, double Fy, or2_solve 
 
     while(s < cfg->max_steps)
     {
-        lbm2(L);
+        if(L) lbm2(L);
         s++;
 
         if(s % cfg->avg_sample == 0)
         {
             double qx, qy;
-            volumeavg2(L, &qx, &qy);
+            flux2(L, seq, nseq, s, &qx, &qy);
             if(!isfinite(qx) || !isfinite(qy)) { out->status = 2; out->fail_step = s; out->steps = s; return; }
             bsx += qx; bsxx += qx*qx;
             bsy += qy; bsyy += qy*qy;
@@ -331,17 +345,17 @@ static void solve2(lat2 *L, const or2_cfg *cfg, double Fx, double Fy, or2_solve 
 
     out->converged = (hold >= cfg->hold) ? 1 : 0;
 
-    snapshot_field2(L);
+    if(L) snapshot_field2(L);
 
     double sx = 0, sy = 0, sxx = 0, syy = 0;
     long   n = 0;
     for(long t = 1; t <= cfg->avg_steps; t++)
     {
-        lbm2(L);
+        if(L) lbm2(L);
         s++;
         if(t % cfg->avg_sample) continue;
         double qx, qy;
-        volumeavg2(L, &qx, &qy);
+        flux2(L, seq, nseq, s, &qx, &qy);
         if(!isfinite(qx) || !isfinite(qy)) { out->status = 2; out->fail_step = s; out->steps = s; return; }
         sx += qx; sxx += qx*qx;
         sy += qy; syy += qy*qy;
@@ -350,7 +364,7 @@ static void solve2(lat2 *L, const or2_cfg *cfg, double Fx, double Fy, or2_solve 
 
     if(n == 0)
     {
-        volumeavg2(L, &out->qx, &out->qy);
+        flux2(L, seq, nseq, s, &out->qx, &out->qy);
         n = 1;
     }
     else
@@ -367,9 +381,9 @@ static void solve2(lat2 *L, const or2_cfg *cfg, double Fx, double Fy, or2_solve 
     }
 
     out->steps    = s;
-    out->fieldres = field_residual2(L);
+    out->fieldres = L ? field_residual2(L) : 0.0;
 
-    if(vel)                                  /* [y][x] rows of (U, V) */
+    if(vel && L)                             /* [y][x] rows of (U, V) */
         for(int y = 0; y < L->LY; y++)
             for(int x = 0; x < L->LX; x++)
             {
@@ -383,9 +397,39 @@ int PFX(solve_dir)(const uint8_t *solid, int LX, int LY, const or2_cfg *cfg,
 {
     lat2 L;
     if(lat2_init(&L, solid, LX, LY, cfg->L0)) { lat2_free(&L); return -1; }
-    solve2(&L, cfg, Fx, Fy, out, vel);
+    solve2(&L, cfg, Fx, Fy, out, vel, NULL, 0);
     lat2_free(&L);
     return out->status;
+}
+
+/* solve()'s convergence rules alone, over a planted flux sequence seq of
+ * nseq x 2 values (the flux after step s is seq[(s-1)*2 ..]); returns 0 ok,
+ * 2 non-finite flux */
+int PFX(replay)(const or2_cfg *cfg, const double *seq, long nseq, or2_solve *out)
+{
+    solve2(NULL, cfg, 0.0, 0.0, out, NULL, seq, nseq);
+    return out->status;
+}
+
+/* the field residual (:169-182) of two [y][x] (U, V) float fields */
+double PFX(field_residual)(int LX, int LY, const float *cur, const float *snap)
+{
+    lat2 L;
+    memset(&L, 0, sizeof(L));
+    L.LX = LX; L.LY = LY;
+    size_t n = (size_t)LX * LY;
+    L.U = (REAL*)malloc(n * sizeof(REAL)); L.V = (REAL*)malloc(n * sizeof(REAL));
+    L.USNAP = (REAL*)malloc(n * sizeof(REAL)); L.VSNAP = (REAL*)malloc(n * sizeof(REAL));
+    if(!L.U || !L.V || !L.USNAP || !L.VSNAP) { lat2_free(&L); return NAN; }
+    for(int y = 0; y < LY; y++)
+        for(int x = 0; x < LX; x++) {
+            size_t d = 2 * ((size_t)y * LX + x);
+            L.U[AT(&L,x,y)] = cur[d];  L.V[AT(&L,x,y)] = cur[d + 1];
+            L.USNAP[AT(&L,x,y)] = snap[d]; L.VSNAP[AT(&L,x,y)] = snap[d + 1];
+        }
+    double r = field_residual2(&L);
+    lat2_free(&L);
+    return r;
 }
 
 /* Run `steps` steps from rest and return df of the buffer the next step reads
@@ -446,9 +490,9 @@ int PFX(solve_tensor)(const uint8_t *solid, int LX, int LY, const or2_cfg *cfg,
     const float mu = PFX(mu)();
     T->mu = mu;
 
-    solve2(&L, cfg, F, 0.0, &T->run[0], NULL);
+    solve2(&L, cfg, F, 0.0, &T->run[0], NULL, NULL, 0);
     if(T->run[0].status) { lat2_free(&L); T->status = 2; return 2; }
-    solve2(&L, cfg, 0.0, F, &T->run[1], NULL);
+    solve2(&L, cfg, 0.0, F, &T->run[1], NULL, NULL, 0);
     if(T->run[1].status) { lat2_free(&L); T->status = 2; return 2; }
 
     const or2_solve *rx = &T->run[0], *ry = &T->run[1];
@@ -475,7 +519,7 @@ int PFX(solve_tensor)(const uint8_t *solid, int LX, int LY, const or2_cfg *cfg,
     {
         const double th = validate_deg * M_PI / 180.0;
         const double Fx = F * cos(th), Fy = F * sin(th);
-        solve2(&L, cfg, Fx, Fy, &T->val, NULL);
+        solve2(&L, cfg, Fx, Fy, &T->val, NULL, NULL, 0);
         if(T->val.status) { lat2_free(&L); T->status = 2; return 2; }
         const double pred_qx = (k_xx * Fx + k_off * Fy) / mu;
         const double pred_qy = (k_off * Fx + k_yy * Fy) / mu;

@@ -281,15 +281,27 @@ static void reset_distributions(lat3 *L)          /* :203-212 */
 
 static inline double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
 
+/* The flux solve() reads after step s: volumeavg of the lattice, or, with no
+ * lattice (L == NULL), entry s of a planted sequence (seq[(s-1)*3 + c]; NaN past
+ * its end), which tests use to drive the convergence rules alone. */
+static void flux3(const lat3 *L, const double *seq, long nseq, long s, double q[3])
+{
+    if(L) { volumeavg(L, q); return; }
+    for(int c = 0; c < 3; c++) q[c] = (s >= 1 && s <= nseq) ? seq[(s - 1) * 3 + c] : NAN;
+}
+
 /* solve(), lbm3d-perm.cpp:320-398, for one forcing vector (fx, fy, fz).
  * vel, when not NULL, receives NN*3 float32 (U,V,W per node, x fastest):
- * the payload export_velocity (:305-317) writes after its 16-byte header. */
+ * the payload export_velocity (:305-317) writes after its 16-byte header.
+ * With L == NULL nothing is stepped and the flux comes from seq (flux3). */
 static void solve_dir(lat3 *L, const or3_cfg *cfg, double fx, double fy, double fz,
-                      or3_solve *out, float *vel)
+                      or3_solve *out, float *vel, const double *seq, long nseq)
 {
     memset(out, 0, sizeof(*out));
-    reset_distributions(L);
-    L->Fx = fx; L->Fy = fy; L->Fz = fz;
+    if(L) {
+        reset_distributions(L);
+        L->Fx = fx; L->Fy = fy; L->Fz = fz;
+    }
 
     long s = 0;
     int hold = 0;
@@ -298,11 +310,12 @@ static void solve_dir(lat3 *L, const or3_cfg *cfg, double fx, double fy, double 
 
     while(s < cfg->max_steps)
     {
-        lbm_step(L); s++;
+        if(L) lbm_step(L);
+        s++;
         if(s % cfg->lag) continue;
 
         double q[3];
-        volumeavg(L, q);
+        flux3(L, seq, nseq, s, q);
         for(int c = 0; c < 3; c++)
             if(!isfinite(q[c])) { out->status = 2; out->fail_step = s; out->steps = s; return; }
 
@@ -325,13 +338,14 @@ static void solve_dir(lat3 *L, const or3_cfg *cfg, double fx, double fy, double 
     long n = 0;
     for(long i = 0; i < cfg->avg_steps; i++)
     {
-        lbm_step(L); s++;
-        double q[3]; volumeavg(L, q);
+        if(L) lbm_step(L);
+        s++;
+        double q[3]; flux3(L, seq, nseq, s, q);
         for(int c = 0; c < 3; c++) { sum[c] += q[c]; sumsq[c] += q[c]*q[c]; }
         n++;
     }
 
-    if(n == 0) volumeavg(L, out->q);
+    if(n == 0) flux3(L, seq, nseq, s, out->q);
     else
         for(int c = 0; c < 3; c++)
         {
@@ -344,7 +358,7 @@ static void solve_dir(lat3 *L, const or3_cfg *cfg, double fx, double fy, double 
         }
     out->steps = s;
 
-    if(vel)
+    if(vel && L)
         for(long i = 0; i < L->nn; i++)
         {
             vel[3*i+0] = (float)L->U[i];
@@ -353,13 +367,22 @@ static void solve_dir(lat3 *L, const or3_cfg *cfg, double fx, double fy, double 
         }
 }
 
+/* solve()'s convergence rules alone, over a planted flux sequence seq of
+ * nseq x 3 values (the flux after step s is seq[(s-1)*3 ..]); returns 0 ok,
+ * 2 non-finite flux */
+int or3_replay(const or3_cfg *cfg, const double *seq, long nseq, or3_solve *out)
+{
+    solve_dir(NULL, cfg, 0, 0, 0, out, NULL, seq, nseq);
+    return out->status;
+}
+
 /* one forcing direction on a mask; returns 0 ok, 2 non-finite flux, -1 OOM */
 int or3_solve_dir(const uint8_t *solid, int nx, int ny, int nz, const or3_cfg *cfg,
                   double fx, double fy, double fz, or3_solve *out, float *vel)
 {
     lat3 L;
     if(lat3_init(&L, solid, nx, ny, nz)) { lat3_free(&L); return -1; }
-    solve_dir(&L, cfg, fx, fy, fz, out, vel);
+    solve_dir(&L, cfg, fx, fy, fz, out, vel, NULL, 0);
     lat3_free(&L);
     return out->status;
 }
@@ -473,11 +496,11 @@ int or3_solve_tensor(const uint8_t *solid, int nx, int ny, int nz, const or3_cfg
     const double F = FX_BASE * cfg->ff;
     T->porosity = porosity; T->force = F; T->mu = MU3;
 
-    solve_dir(&L, cfg, F, 0, 0, &T->run[0], NULL);
+    solve_dir(&L, cfg, F, 0, 0, &T->run[0], NULL, NULL, 0);
     if(T->run[0].status) { lat3_free(&L); T->status = 2; return 2; }
-    solve_dir(&L, cfg, 0, F, 0, &T->run[1], NULL);
+    solve_dir(&L, cfg, 0, F, 0, &T->run[1], NULL, NULL, 0);
     if(T->run[1].status) { lat3_free(&L); T->status = 2; return 2; }
-    solve_dir(&L, cfg, 0, 0, F, &T->run[2], NULL);
+    solve_dir(&L, cfg, 0, 0, F, &T->run[2], NULL, NULL, 0);
     if(T->run[2].status) { lat3_free(&L); T->status = 2; return 2; }
 
     for(int c = 0; c < 3; c++)
@@ -511,7 +534,7 @@ int or3_solve_tensor(const uint8_t *solid, int nx, int ny, int nz, const or3_cfg
     if(validate)
     {
         const double c = 1.0/sqrt(3.0);
-        solve_dir(&L, cfg, F*c, F*c, F*c, &T->val, NULL);
+        solve_dir(&L, cfg, F*c, F*c, F*c, &T->val, NULL, NULL, 0);
         if(T->val.status) { lat3_free(&L); T->status = 2; return 2; }
         double err = 0, qn = 0;
         for(int r = 0; r < 3; r++)

@@ -61,6 +61,33 @@ def assert_printed_close(ours, ref, floor=0.0, what=""):
     assert abs(ours - ref) <= tol, f"{what}: ours {ours!r} ref {ref!r} |d|={abs(ours-ref):.3e} tol {tol:.3e}"
 
 
+def check_eigen_and_reciprocity(get, row, trace, what=""):
+    """The CSV's principal vectors v1..v3, fa and recip_* (LMB3d/lbm3d-perm.cpp:
+    531-553, 605-613) against a reference row.  get(key) gives our value.
+
+    A principal vector is defined (with the sign jacobi3 gives it) only when
+    its value is separated from the other two: K known to ~1e-10 trace per
+    component moves it by about |dK| / gap, so it is compared with that bound
+    (plus its printed precision) and skipped when the gap is below 1e-6 trace
+    (a degenerate pair, e.g. K_xx = K_yy of a slit, has no preferred basis).
+    fa is a smooth function of the principal values (1e-9); recip_* are
+    differences of off-diagonals over the trace (3e-10 absolute)."""
+    k = [float(row["k1"]), float(row["k2"]), float(row["k3"])]
+    checked = 0
+    for j in range(3):
+        gap = min(abs(k[j] - k[i]) for i in range(3) if i != j)
+        if gap < 1e-6 * trace:
+            continue
+        for a in "xyz":
+            key = f"v{j + 1}_{a}"
+            assert_printed_close(float(get(key)), float(row[key]), 1e-9 * trace / gap, f"{what} {key}")
+        checked += 1
+    assert_printed_close(float(get("fa")), float(row["fa"]), 1e-9, f"{what} fa")
+    for key in ("recip_xy", "recip_xz", "recip_yz", "recip_resid"):
+        assert_printed_close(float(get(key)), float(row[key]), 3e-10, f"{what} {key}")
+    return checked
+
+
 # GOLDEN_FAST: also re-solved by the CPU oracle (seconds each); rows of 70 cells
 # (a segment boundary) and a porosity-0.3 field.  GOLDEN_ALL adds 130-cell rows
 # and the longer cases, compared with the reference rows only.

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, REPO, assert_printed_close, golden
+from conftest import GOLDEN, REPO, assert_printed_close, check_eigen_and_reciprocity, golden
 
 BIN = REPO / "flow-anizotropy_amd" / "bin"
 
@@ -82,6 +82,7 @@ def test_lbm3d_row_matches_the_reference_row(tmp_path):
             assert ours[k] == v, k
         elif k.startswith(("K_", "k_", "k1", "k2", "k3")):
             assert_printed_close(float(ours[k]), float(v), 1e-10 * trace, k)
+    assert check_eigen_and_reciprocity(ours.__getitem__, refd, trace, name) >= 1
     rc2, _, _ = run("lbm3d-perm", GOLDEN / f"{name}.raw", out, *argv)
     assert rc2 == rc and len(out.read_text().splitlines()) == 3
 

@@ -11,7 +11,8 @@ to the component or, for near-zero off-diagonals, to the trace.
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_ALL, GOLDEN_FAST, assert_printed_close, golden, golden_flags
+from conftest import (GOLDEN_ALL, GOLDEN_FAST, assert_printed_close, check_eigen_and_reciprocity,
+                      golden, golden_flags)
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +45,10 @@ def test_fp64_bit_exact_against_oracle(gpu_ctx, oracle, name):
             assert r[f"K_{a}{b}"] == K[i, j]
     assert r["k1"] == T.kval[0] and r["k3"] == T.kval[2] and r["fa"] == T.fa
     assert r["recip_resid"] == T.recip_resid
+    vec = np.array(T.vec[:]).reshape(3, 3)
+    for j in range(3):
+        for i, a in enumerate(AX):
+            assert r[f"v{j + 1}_{a}"] == vec[i, j]
 
 
 @pytest.mark.parametrize("name", GOLDEN_ALL)
@@ -79,6 +84,7 @@ def test_fp64_matches_reference_rows(gpu_ctx, name):
     for key in ("k_xy", "k_xz", "k_yz", "k1", "k2", "k3", "k_mean"):
         assert_printed_close(r[key], row[key], 1e-10 * trace, key)
     assert_printed_close(r["porosity"], row["porosity"], 0, "porosity")
+    check_eigen_and_reciprocity(r.__getitem__, row, trace, name)
     if "--validate" in meta["argv"]:
         # the binary prints the tensor test on stdout (:566-571)
         line = [l for l in meta["stdout"] if "relative error" in l][0]

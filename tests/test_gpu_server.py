@@ -137,6 +137,8 @@ def test_server_2d_equals_plain_solve(gpu_ctx):
 
 
 def test_poisson_schedule_latencies(gpu_ctx):
+    """Requests arriving on a Poisson schedule while others step: timestamps
+    in order, and every row equal bit for bit to the same request alone."""
     from flowperm import masks
     from flowperm.server import Server, poisson_arrivals, serve_schedule
     ms = [masks.spheres((n, n, n), 0.6, 3, 5, seed=i) for i, n in enumerate((16, 20, 24, 16, 28))]
@@ -145,8 +147,105 @@ def test_poisson_schedule_latencies(gpu_ctx):
         out = serve_schedule(s, ms, arr, CFG)
     lat = [o["latency"] for o in out]
     print("latencies", [round(x, 4) for x in lat], flush=True)
-    for o in out:
+    for m, o in zip(ms, out):
         assert o["t_submit"] >= o["arrival"] - 1e-6
         assert o["t_done"] >= o["t_admit"] >= o["t_submit"]
         assert o["latency"] > 0
         assert o["row"]["conv_fx"] == 1
+        ref = alone(gpu_ctx, m, "f32")[0]
+        for k in KEYS3:
+            assert o["row"][k] == ref[k], k
+
+
+def _footprint(gpu_ctx, m):
+    from flowperm.server import Server
+    with Server(3, "f32", ctx=gpu_ctx) as probe:
+        probe.submit(m, **CFG)
+        probe.step()
+        b = probe.stats()["bytes_in_use"]
+        probe.run()
+    return b
+
+
+def test_small_requests_pass_a_waiting_large_one_up_to_the_bypass_limit(gpu_ctx):
+    """A request that does not fit yet is passed by later ones that do (they
+    need no memory it is waiting for), at most max_bypass times; then arrival
+    order holds again.  Every result is the one the request gives alone."""
+    from flowperm import masks
+    from flowperm.server import Server
+    big = [masks.spheres((40, 40, 40), 0.6, 3, 5, seed=s) for s in (11, 12)]
+    small = [masks.spheres((16, 16, 16), 0.6, 3, 5, seed=s) for s in (13, 14)]
+    nb, ns = _footprint(gpu_ctx, big[0]), max(_footprint(gpu_ctx, m) for m in small)
+    assert nb > 3 * ns
+    budget = nb + 2 * ns + 4096                   # one large + two small, not two large
+    order = [big[0], big[1], small[0], small[1]]
+    ref = [alone(gpu_ctx, m, "f32")[0] for m in order]
+    for bypass, want_resident in ((1, 2), (4, 3)):
+        with Server(3, "f32", budget_bytes=budget, ctx=gpu_ctx, max_bypass=bypass) as s:
+            ids = [s.submit(m, **CFG) for m in order]
+            s.step()
+            st = s.stats()
+            print(f"max_bypass {bypass}: resident {st['resident']} queued {st['queued']}", flush=True)
+            # bypass 1: the first small request passes the waiting large one, the
+            # second may not; bypass 4: both pass it
+            assert st["resident"] == want_resident
+            res = s.run(ids)
+        info = [res[i][1] for i in ids]
+        assert info[2]["t_admit"] < info[1]["t_admit"]
+        if bypass == 1:
+            assert info[3]["t_admit"] >= info[1]["t_admit"]
+        for i, rid in enumerate(ids):
+            for k in KEYS3:
+                assert res[rid][0][k] == ref[i][k], (bypass, i, k)
+
+
+def test_strict_arrival_order_when_asked(gpu_ctx):
+    from flowperm import masks
+    from flowperm.server import Server
+    big = masks.spheres((40, 40, 40), 0.6, 3, 5, seed=11)
+    small = masks.spheres((16, 16, 16), 0.6, 3, 5, seed=13)
+    nb = _footprint(gpu_ctx, big)
+    with Server(3, "f32", budget_bytes=nb + 4096 + nb // 4, ctx=gpu_ctx, max_bypass=-1) as s:
+        ids = [s.submit(big, **CFG), s.submit(big, **CFG), s.submit(small, **CFG)]
+        s.step()
+        assert s.stats()["resident"] == 1 and s.stats()["queued"] == 2
+        res = s.run(ids)
+    assert res[ids[2]][1]["t_admit"] >= res[ids[1]][1]["t_admit"]
+
+
+def test_failed_chunk_ends_resident_requests_and_the_server_goes_on(gpu_ctx):
+    """A chunk that fails (a table rebuild forced to fail, as a device error
+    would, fp_test_fail_rebuilds) ends every resident request with
+    FP_ERR_DEVICE and returns its memory.  Nothing of those requests stays in
+    the batch: stepping again is idle, a failed request polled late keeps its
+    status, and a new request solves exactly as alone."""
+    from flowperm import _lib
+    from flowperm._lib import FlowpermError
+    from flowperm.server import Server
+    ms = masks3d()[:2]
+    ref = alone(gpu_ctx, ms[0], "f32")[0]
+    lib = gpu_ctx.lib
+    with Server(3, "f32", ctx=gpu_ctx) as s:
+        ids = [s.submit(m, **CFG) for m in ms]
+        lib.fp_test_fail_rebuilds(1)
+        try:
+            with pytest.raises(FlowpermError) as e:
+                s.step()
+        finally:
+            lib.fp_test_fail_rebuilds(0)
+        assert e.value.status == _lib.FP_ERR_DEVICE
+        st = s.stats()
+        assert st["bytes_in_use"] == 0 and st["resident"] == 0 and st["queued"] == 0
+        assert s.step() == 0
+        with pytest.raises(FlowpermError) as e0:
+            s.poll(ids[0])
+        assert e0.value.status == _lib.FP_ERR_DEVICE
+        rid = s.submit(ms[0], **CFG)
+        s.step()
+        res = s.run([rid])
+        with pytest.raises(FlowpermError) as e1:
+            s.poll(ids[1])
+        assert e1.value.status == _lib.FP_ERR_DEVICE
+        assert s.stats()["bytes_in_use"] == 0
+    for k in KEYS3:
+        assert res[rid][0][k] == ref[k], k

@@ -61,6 +61,89 @@ def test_gif_round_trip(tmp_path):
     assert np.array_equal(structio.load_structure2d(tmp_path / "m.gif"), m)
 
 
+def test_atoi_semantics():
+    """the C library's atoi on one extracted token (LMB2d/lbm.cpp:443, :464)"""
+    from flowperm.structio import atoi
+    for tok, want in [("0", 0), ("255", 255), ("12abc", 12), ("abc", 0), ("-3", -3),
+                      ("+7", 7), ("007", 7), ("0.5", 0), ("3.9", 3), ("-0", 0), ("+-1", 0),
+                      ("0x1F", 0), ("", 0), ("1e3", 1)]:
+        assert atoi(tok) == want, tok
+
+
+# a 5 x 3 PPM in the reference's layout: four header lines (read_from_ppm
+# skips them without parsing), then rows y = 0..LY-1 of x = 0..LX-1 (R, G, B)
+# tokens; pore where atoi(R) != 0 -- "12abc", "-3" and "+1" are pores, "abc",
+# "0" and "0.9" are solid
+PPM_R = [["255", "0", "12abc", "abc", "-3"],
+         ["0", "0", "0.9", "+1", "255"],
+         ["7", "0", "0", "0", "1"]]
+
+
+def _ppm_text(rows, comment_first=False):
+    head = ["P3", "# Created by IrfanView", "5 3", "255"] if comment_first else \
+           ["P3", "5 3", "# Created by IrfanView", "255"]
+    body = []
+    for row in rows:
+        body.append(" ".join(f"{r} 17 0" for r in row))
+    return "\n".join(head + body) + "\n"
+
+
+@pytest.mark.parametrize("comment_first", [False, True])
+def test_ppm_reader_follows_read_from_ppm(tmp_path, comment_first):
+    from flowperm import structio
+    p = tmp_path / "m.ppm"
+    p.write_text(_ppm_text(PPM_R, comment_first))
+    m = structio.load_structure2d(p)
+    want = np.array([[0 if structio.atoi(r) != 0 else 1 for r in row] for row in PPM_R], np.uint8)
+    assert m.shape == (3, 5) and m.dtype == np.uint8
+    assert np.array_equal(m, want)
+    assert want.tolist() == [[0, 1, 0, 1, 0], [1, 1, 1, 0, 0], [0, 1, 1, 1, 0]]
+    # values wrapped over lines other than one row per line read the same
+    p.write_text(_ppm_text(PPM_R, comment_first).replace(" 17 0 ", "\n17\n0\n"))
+    assert np.array_equal(structio.load_structure2d(p), want)
+    # fewer triples than the header's lattice
+    p.write_text(_ppm_text(PPM_R[:2], comment_first))
+    with pytest.raises(structio.StructureError, match="truncated"):
+        structio.load_structure2d(p)
+
+
+def test_dat_reader_follows_read_from_dat(tmp_path):
+    """pore where atoi(token) == 0 (LMB2d/lbm.cpp:456-470); any other file
+    extension than .ppm / .gif is read as DAT (:472-483)"""
+    from flowperm import structio
+    toks = [["0", "1", "2", "0.5", "-1"],
+            ["abc", "0", "00", "3x", "+0"],
+            ["1", "1", "0", "9", "0"]]
+    want = np.array([[1 if structio.atoi(t) != 0 else 0 for t in row] for row in toks], np.uint8)
+    assert want.tolist() == [[0, 1, 1, 0, 1], [0, 0, 0, 1, 0], [1, 1, 0, 1, 0]]
+    p = tmp_path / "m.txt"
+    p.write_text("\n".join(" ".join(r) for r in toks) + "\n")
+    assert np.array_equal(structio.load_structure2d(p, shape=(3, 5)), want)
+    with pytest.raises(structio.StructureError, match="square"):
+        structio.load_structure2d(p)
+    q = tmp_path / "sq.dat"
+    q.write_text(" ".join(["0", "1", "0", "1"] * 4))
+    assert structio.load_structure2d(q).tolist() == [[0, 1, 0, 1]] * 4
+    with pytest.raises(structio.StructureError, match="truncated"):
+        structio.load_structure2d(p, shape=(4, 5))
+
+
+def test_grf_cluster_porosity_hits_its_target():
+    """the Gaussian random field's kept percolating cluster, not the threshold
+    level, has the requested porosity when cluster_porosity is asked for.  Below
+    the level at which a cluster first wraps along every axis no threshold can
+    give the target: then the percolating cluster closest to it comes back."""
+    from flowperm import masks
+    for por, seed in ((0.45, 1), (0.55, 2), (0.25, 5)):
+        m = masks.grf((48, 48, 48), porosity=por, corr=(4.0, 4.0, 8.0), seed=seed,
+                      require_all=True, cluster_porosity=True)
+        assert abs((1 - m.mean()) - por) <= 0.002, (por, seed, 1 - m.mean())
+        assert all(masks.percolates(m))
+    m = masks.grf((48, 48, 48), porosity=0.2, corr=(4.0, 4.0, 8.0), seed=1, require_all=True,
+                  cluster_porosity=True)
+    assert all(masks.percolates(m)) and 0.2 < 1 - m.mean() < 0.3
+
+
 def _brute_wraps(sol):
     """BFS over the lifted torus: an axis wraps when a cluster reaches a copy
     of one of its own cells displaced along it"""

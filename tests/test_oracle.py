@@ -16,7 +16,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_FAST, assert_printed_close, golden, golden_flags
+from conftest import GOLDEN_FAST, assert_printed_close, check_eigen_and_reciprocity, golden, golden_flags
 
 pytestmark = pytest.mark.filterwarnings("ignore")
 
@@ -88,6 +88,11 @@ def _check_tensor_against_row(T, row, what):
         assert_printed_close(T.kval[j], row[f"k{j + 1}"], 1e-10 * trace, f"{what} k{j + 1}")
     assert_printed_close(T.porosity, row["porosity"], 0, f"{what} porosity")
     assert row["nx"] == T.nx and row["ny"] == T.ny and row["nz"] == T.nz
+    vec = np.array(T.vec[:]).reshape(3, 3)             # vec[component][eigen index]
+    ours = {f"v{j + 1}_{a}": vec[i, j] for j in range(3) for i, a in enumerate("xyz")}
+    ours.update(fa=T.fa, recip_xy=T.recip_xy, recip_xz=T.recip_xz, recip_yz=T.recip_yz,
+                recip_resid=T.recip_resid)
+    return check_eigen_and_reciprocity(ours.__getitem__, row, trace, what)
 
 
 @pytest.mark.parametrize("name", GOLDEN_FAST + ["sph_x28y24z20_s1"])
@@ -95,7 +100,8 @@ def test_oracle_matches_reference_rows(oracle, name):
     mask, row, meta = golden(name)
     rc, T = oracle.solve_tensor3d(mask, **golden_flags(meta))
     assert rc == meta["returncode"]
-    _check_tensor_against_row(T, row, name)
+    n_vectors = _check_tensor_against_row(T, row, name)
+    assert n_vectors >= 1, "no principal vector separated enough to compare"
 
 
 def test_oracle_error_codes_match_reference(oracle):
