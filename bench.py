@@ -58,18 +58,22 @@ def workload_mask(n, seed, kind="ellipsoids"):
     return m, por
 
 
+TRAFFIC_PROFILE = "profiles/d3q19_f32_step_kernels_256cube_ellipsoids.md"
+
+
 def measured_traffic(n_fluid):
-    """HBM bytes per step-kernel launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this lattice (copy-calibrated), or None when the
-    summary is for another lattice."""
-    p = REPO / "profiles" / "d3q19_f32_step_kernels_256cube_ellipsoids.md"
+    """HBM bytes per launch of each step kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this lattice (copy-calibrated counter
+    runs of bench.py, not measured in this run), or None when the summary is
+    for another lattice."""
+    p = REPO / TRAFFIC_PROFILE
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
         if line.startswith("traffic-json:"):
             d = json.loads(line.split(":", 1)[1])
             if d["n_fluid"] == n_fluid:
-                return d["bytes_per_launch"]
+                return d
     return None
 
 
@@ -182,43 +186,61 @@ def batched_full(ctx, n, m, seed, workers):
             "max_k_error_first_m": err}
 
 
-def batched_sample(ctx, n, seed, workers):
-    """n structures of the batched set: all 3n forcing runs stepped together
-    in FP32 by one request server, against the plain FP64 path on the same n"""
+def batched_sample(ctx, n, m, seed, workers, fp64_gbs=None):
+    """The first n structures of the batched set (512 in all): all 3n forcing
+    runs stepped together in FP32 by one request server, against the plain
+    FP64 path (one structure at a time) on the first m of them.  The speedup
+    compares the plain FP64 time of those m with the time their updates take
+    at the batch's aggregate rate."""
     from flowperm import workloads
     t0 = time.time()
     structs = workloads.generate("batched", n, seed, workers)
     edges = [int(s.shape[0]) for s in structs]
     log(f"batched: {n} structures, edges {edges} ({time.time() - t0:.1f} s to generate)")
     rows32, t32, st = burst(ctx, structs)
+    rate32 = st["fluid_updates"] / t32
     log(f"batched FP32: {n} tensors in {t32:.2f} s = {n / t32:.3f} tensors/s; "
-        f"{st['fluid_updates'] / t32 / 1e6:.0f} MLUPS, {st['chunks']} chunks")
-    rows64, t64 = plain_fp64(ctx, structs)
+        f"{rate32 / 1e6:.0f} MLUPS, {st['chunks']} chunks")
+    rows64, t64 = plain_fp64(ctx, structs[:m])
     upd64 = sum(r["fluid_updates"] for r in rows64)
-    log(f"plain FP64: {n} tensors in {t64:.2f} s = {n / t64:.3f} tensors/s; "
-        f"{upd64 / t64 / 1e6:.0f} MLUPS")
-    err = max(k_error(a, b) for a, b in zip(rows32, rows64))
+    t32m = sum(r["fluid_updates"] for r in rows32[:m]) / rate32
+    log(f"plain FP64: {m} tensors in {t64:.2f} s = {m / t64:.3f} tensors/s; "
+        f"{upd64 / t64 / 1e6:.0f} MLUPS; speedup on them {t64 / t32m:.2f}x")
+    err = max(k_error(a, b) for a, b in zip(rows32[:m], rows64))
     fluid = [int((s == 0).sum()) for s in structs]
     steps32 = [max(r["steps_fx"], r["steps_fy"], r["steps_fz"]) for r in rows32]
     same_steps = sum(int(all(a[f"steps_{t}"] == b[f"steps_{t}"] for t in ("fx", "fy", "fz")))
-                     for a, b in zip(rows32, rows64))
-    return {
-        "structures": n, "edges": edges, "mean_fluid_cells": float(np.mean(fluid)),
+                     for a, b in zip(rows32[:m], rows64))
+    out = {
+        "structures": n, "of_set": 512, "edges": edges, "mean_fluid_cells": float(np.mean(fluid)),
         "instances": 3 * n, "config": "ellipsoids 3:1, 15 deg jitter, porosity U(0.3, 0.8); "
                                       "--ff 40 --lag 1000 --eps 1e-6",
+        "excluded": f"structures {n}..511 of the 512-structure set (bench.py --study batched-full "
+                    f"solves all of them)",
         "batched_fp32_tensors_per_s": n / t32, "batched_fp32_seconds": t32,
-        "batched_fp32_mlups": st["fluid_updates"] / t32 / 1e6,
-        "plain_fp64_tensors_per_s": n / t64, "plain_fp64_seconds": t64,
-        "plain_fp64_mlups": upd64 / t64 / 1e6,
-        "speedup": t64 / t32,
+        "batched_fp32_mlups": rate32 / 1e6,
+        "plain_fp64_structures": m, "plain_fp64_tensors_per_s": m / t64,
+        "plain_fp64_seconds": t64, "plain_fp64_mlups": upd64 / t64 / 1e6,
+        "speedup": t64 / t32m,
+        "speedup_note": "plain FP64 time of the first m structures over the time their updates "
+                        "take at the batched FP32 aggregate rate",
         "max_k_error_vs_fp64": err, "structures_with_fp64_step_counts": same_steps,
         "max_steps": int(max(steps32)), "median_steps": float(np.median(steps32))}
+    if fp64_gbs:
+        # FP32 moves half the bytes per update: a bandwidth-bound FP32 batch is
+        # 2 x (its bandwidth / FP64's) faster, so 3x needs 1.5x FP64's bandwidth
+        out["bound_for_3x"] = {
+            "fp64_step_kernel_GBs": round(fp64_gbs, 1),
+            "fp32_GBs_needed": round(1.5 * fp64_gbs, 1),
+            "fp32_frac_of_peak_needed": round(1.5 * fp64_gbs / HBM_PEAK_GBS, 3),
+            "note": "speedup = 2 x BW(FP32) / BW(FP64) x batching gain on bandwidth-bound steps"}
+    return out
 
 
 def serving_sample(ctx, n, seed, workers, load=0.8):
     """n requests of the serving set: the burst capacity (all submitted at
     once), then Poisson arrivals at load x that capacity with latencies, and
-    the plain FP64 sequential path on the same n"""
+    the plain FP64 sequential path on the same n requests"""
     from flowperm import workloads
     from flowperm.server import Server, poisson_arrivals, serve_schedule
     t0 = time.time()
@@ -563,9 +585,11 @@ def main():
     ap.add_argument("--tune", default="", help="fixed launch shapes "
                     "'cpt,waves,nt,local_bpc,nbr_bpc[,nbr_mode]' for both precisions instead of "
                     "running the autotuner")
-    ap.add_argument("--batched", type=int, default=12, help="structures in the batched-solve "
-                    "sample, solved batched FP32 and plain FP64 (0 = skip)")
-    ap.add_argument("--serving", type=int, default=10, help="requests in the serving sample (0 = skip)")
+    ap.add_argument("--batched", type=int, default=48, help="structures of the batched set "
+                    "solved batched in FP32 (0 = skip)")
+    ap.add_argument("--batched-fp64", type=int, default=12, help="of those, how many the plain "
+                    "FP64 path solves one at a time for the speedup and the K error")
+    ap.add_argument("--serving", type=int, default=40, help="requests in the serving sample (0 = skip)")
     ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
     ap.add_argument("--study", default="", choices=["", "precision", "large", "large-solve",
                                                      "batched-full", "small-batched"],
@@ -601,8 +625,10 @@ def main():
     mask, por_target = workload_mask(a.size, a.seed, a.mask)
     n_fluid = int(mask.size - np.count_nonzero(mask))
     ctx = solver.Context()
-    steps = max(2, a.steps & ~1)
-    warm = max(0, a.warmup & ~1)
+    # a step pair is one local and one neighbour step: even counts (an odd
+    # request is rounded up, and the line reports what ran)
+    steps = max(2, (a.steps + 1) & ~1)
+    warm = max(0, (a.warmup + 1) & ~1)
 
     from flowperm import autotune
     tuned = {}
@@ -634,9 +660,13 @@ def main():
     kern_ms = 0.5 * (o32.local_ms + o32.nbr_ms)
     bytes_launch = o32.n_fluid * o32.bytes_per_update
     achieved = bytes_launch / (kern_ms / 1e3) / 1e9
+    gbs = {"k_local": bytes_launch / (o32.local_ms / 1e3) / 1e9,
+           "k_nbr": bytes_launch / (o32.nbr_ms / 1e3) / 1e9}
+    dominant = "k_nbr" if o32.nbr_ms >= o32.local_ms else "k_local"
     log(f"FP32: {mlups:.1f} MLUPS, {ms_step:.4f} ms/step as graph replays "
-        f"({o32.eager_total_ms / o32.steps:.4f} launched one by one); local {o32.local_ms:.4f} ms, "
-        f"neighbour {o32.nbr_ms:.4f} ms, monitor {o32.monitor_ms:.4f} ms; "
+        f"({o32.eager_total_ms / o32.steps:.4f} launched one by one); local {o32.local_ms:.4f} ms "
+        f"({gbs['k_local'] / HBM_PEAK_GBS:.3f} of peak), neighbour {o32.nbr_ms:.4f} ms "
+        f"({gbs['k_nbr'] / HBM_PEAK_GBS:.3f}), monitor {o32.monitor_ms:.4f} ms; "
         f"step kernels {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.3f} of peak")
 
     extra = {}
@@ -665,11 +695,14 @@ def main():
             "step_kernel_GBs": o64.n_fluid * o64.bytes_per_update / (k64 / 1e3) / 1e9}
         log(f"FP64: {extra['fp64']['mlups']:.1f} MLUPS, {extra['fp64']['step_kernel_GBs']:.0f} GB/s")
     if a.batched > 0:
-        extra["batched"] = batched_sample(ctx, a.batched, a.seed, a.workers)
+        fp64_gbs = extra.get("fp64", {}).get("step_kernel_GBs")
+        extra["batched"] = batched_sample(ctx, a.batched, min(a.batched, a.batched_fp64), a.seed,
+                                          a.workers, fp64_gbs)
     if a.serving > 0:
         extra["serving"] = serving_sample(ctx, a.serving, a.seed, a.workers)
     ctx.close()
 
+    traffic = measured_traffic(n_fluid) if a.mask == "ellipsoids" else None
     cpu = None
     if not a.no_cpu:
         log("CPU baseline: reference solver on this host ...")
@@ -686,6 +719,7 @@ def main():
         "n_gpus": 1,
         "steps": int(o32.steps),
         "warmup": warm,
+        "warmup_requested": a.warmup,
         "ms_per_step": round(ms_step, 5),
         "eager_ms_per_step": round(o32.eager_total_ms / o32.steps, 5),
         "higher_is_better": True,
@@ -700,16 +734,27 @@ def main():
             "lattice": [a.size] * 3, "n_fluid": n_fluid, "n_cells": int(mask.size),
             "force": "2.5e-7 * 40 along x", "parallelism": "single GPU"},
         "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n_fluid) if a.mask == "ellipsoids" else None,
-            "traffic_note": "HBM bytes per launch (mean of the two step kernels) from rocprofv3 "
-                            "FETCH_SIZE x2 + WRITE_SIZE x1 (factors from a copy of known size); "
-                            f"algorithmic {bytes_launch} B",
-            "kernel": "k_local/k_nbr<D3Q19,F32>: 152 B per fluid update x n_fluid per launch "
-                      "over the mean event-timed duration of the two step kernels",
-            "local_ms": round(o32.local_ms, 5), "neighbour_ms": round(o32.nbr_ms, 5),
-            "monitor_ms": round(o32.monitor_ms, 5)},
+            "bound": "hbm", "achieved": round(gbs[dominant], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs[dominant] / HBM_PEAK_GBS, 4),
+            "traffic": (traffic[{"k_nbr": "nbr_bytes", "k_local": "local_bytes"}[dominant]]
+                        if traffic else None),
+            "kernel": f"{dominant}<D3Q19,F32> (the dominant step kernel: the longer of the two, "
+                      f"one launch per step each): 152 B per fluid update x {n_fluid} fluid "
+                      f"cells = {bytes_launch:.0f} B per launch over its mean event-timed duration",
+            "traffic_source": (f"{TRAFFIC_PROFILE}: rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
+                               f"(factors from a copy of known size), committed counter runs of "
+                               f"this lattice, not measured in this run") if traffic else None,
+            "kernels": {
+                "k_local": {"ms": round(o32.local_ms, 5), "achieved": round(gbs["k_local"], 1),
+                            "frac": round(gbs["k_local"] / HBM_PEAK_GBS, 4),
+                            "traffic": traffic["local_bytes"] if traffic else None},
+                "k_nbr": {"ms": round(o32.nbr_ms, 5), "achieved": round(gbs["k_nbr"], 1),
+                          "frac": round(gbs["k_nbr"] / HBM_PEAK_GBS, 4),
+                          "traffic": traffic["nbr_bytes"] if traffic else None},
+                "step_mean": {"ms": round(kern_ms, 5), "achieved": round(achieved, 1),
+                              "frac": round(achieved / HBM_PEAK_GBS, 4)},
+                "k_finish_ms": round(o32.monitor_ms, 5)},
+            "copy_GBs": extra.get("copy_GBs")},
         "autotune": tuned,
         "cpu_baseline": cpu,
     }

@@ -15,7 +15,10 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
 // offsets in the neighbour kernels)
 void kern_step(int lattice, int prec, bool local, const Tuning &tu, bool b32, const InstDesc *insts,
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
-void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s);
+// plain copies of n16 float4: kind 0 persistent grid, 1 one workgroup per
+// 8 KiB, 2 the same with non-temporal loads and stores
+constexpr int N_COPY_KINDS = 3;
+void kern_copy(int kind, const void *a, void *b, int64_t n16, hipStream_t s);
 // after a step: exact flux sums and the convergence update of every instance
 void kern_finish(int nwork, const RedWork *work, const InstDesc *insts, const InstCfg *cfgs,
                  InstState *st, const Partial *part, RPartial *red, unsigned int *counter,
@@ -174,6 +177,22 @@ private:
     hipGraphExec_t exec_ = nullptr;
     BatchStats stats_;
 };
+
+// The work lists, flux-partial slots and reduction items of a batch (the
+// host half of a table rebuild), and their check.
+struct BatchTables {
+    std::vector<InstDesc> desc;
+    std::vector<LocalWork> lw, cw;       // local items; compact neighbour items
+    std::vector<NbrWork> nw;             // row-segment neighbour items
+    std::vector<RedWork> lr, nr;         // reduction items over local / neighbour partials
+};
+void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const Tuning &tu, BatchTables &t);
+int64_t check_tables(const BatchTables &t, std::string *first);
+// host only: the tables of the structures' forcing runs as a batch would build
+// them, checked (*bad violations); returns the first problem, or a mask error
+std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,
+                                const std::vector<int> &dims, int lattice, int prec,
+                                const Tuning &tu, int64_t *bad);
 
 // the D3Q19 FP32 per-direction force terms of InstDesc
 void f32_force_terms(const double F[3], float g45[19], float h15[19]);

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 namespace fp {
 
@@ -484,6 +485,232 @@ std::string Batch::admit(Job *j)
     return "";
 }
 
+// The host half of a table rebuild, for jobs whose neighbour-step form is
+// decided (Job::nbr_mode): every kernel's work items, the flux-partial slots
+// of each instance (InstDesc lp0..lp1 in the local partials, np0..np1 in the
+// neighbour partials, where the row-segment items' slots come first and the
+// compact items' follow), and the reduction items over those slots.  Sets each
+// job's item count.
+void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const Tuning &tu, BatchTables &t)
+{
+    const int n_inst = (int)jobs.size();
+    std::vector<InstDesc> &desc = t.desc;
+    std::vector<LocalWork> &lw = t.lw, &cw = t.cw;
+    std::vector<NbrWork> &nw = t.nw;
+    desc.assign((size_t)n_inst, InstDesc{});
+    lw.clear(); cw.clear(); nw.clear();
+    for(int i = 0; i < n_inst; i++) {
+        Job *j = jobs[i];
+        desc[i] = make_desc(*j);
+        const Geometry &g = *j->geo;
+        const int64_t lchunk = 256 * (int64_t)tu.local_cpt;
+        desc[i].lp0 = (int32_t)lw.size();
+        for(int64_t c = 0; c < g.n_fluid; c += lchunk) {
+            LocalWork w;
+            w.inst = i;
+            w.ncell = (int32_t)std::min<int64_t>(lchunk, g.n_fluid - c);
+            w.first = c;
+            lw.push_back(w);
+        }
+        desc[i].lp1 = (int32_t)lw.size();
+        // large lattices deal their neighbour-step items to the XCDs in y bands
+        // (see below)
+        const bool banded = g.dim == 3 &&
+            (double)lattice_q(lattice) * (double)j->stride * (double)elem_bytes(prec) >= BAND_MIN_BYTES;
+        if(j->nbr_mode == 1) {
+            // compact neighbour step: up to 256 consecutive fluid cells per
+            // item; its partial slots follow the row-segment ones (offset
+            // fixed below).  Banded: the cells of rows y0..y1-1 of one plane
+            // are consecutive, so band b is plane after plane of such runs
+            // (256^3 FP32 0.385 against 0.406 ms, FP64 0.653 against 0.761;
+            // 512^3 GRF FP32 0.700 against 0.712).
+            desc[i].np0 = (int32_t)cw.size();
+            auto runs = [&](int64_t a0, int64_t a1) {
+                for(int64_t c = a0; c < a1; c += 256) {
+                    LocalWork w;
+                    w.inst = i;
+                    w.ncell = (int32_t)std::min<int64_t>(256, a1 - c);
+                    w.first = c;
+                    cw.push_back(w);
+                }
+            };
+            if(banded) {
+                const int64_t band = (g.ny + 7) / 8;
+                for(int64_t y0 = 0; y0 < g.ny; y0 += band) {
+                    const int64_t y1 = std::min<int64_t>(g.ny, y0 + band);
+                    for(int64_t z = 0; z < g.nz; z++)
+                        runs(g.row_first[(size_t)(z * g.ny + y0)], g.row_first[(size_t)(z * g.ny + y1)]);
+                }
+            } else {
+                runs(0, g.n_fluid);
+            }
+            desc[i].np1 = (int32_t)cw.size();
+            continue;
+        }
+        desc[i].np0 = (int32_t)nw.size();
+        {
+            // slabs of nbr_waves rows stacked along z (2D: y); inside a slab y
+            // then the segment vary fastest.  The kernels split their items
+            // into 8 contiguous ranges, one per XCD.  Large lattices deal the
+            // rows in 8 bands of ny/8 rows, each band through all z, slab after
+            // slab: an XCD then owns one band and walks it in z, so the rows
+            // either side of a slab boundary are in flight together and share
+            // its L2 (256^3 FP32 step 0.318 against 0.332 ms; bands of 8-64
+            // rows inside per-XCD z ranges gave 0.327-0.331).  Below ~400 MB of
+            // populations the plain z order stays: an XCD's neighbour-step
+            // range then covers the same cells as its local-step range, and
+            // the L2 lines the one kernel leaves are the other's (192^3 FP32:
+            // bands 0.139, z order 0.135 ms; 96^3: 0.030 against 0.027).
+            const int64_t nwv = tu.nbr_waves;
+            const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
+            const int64_t ys = g.dim == 3 ? g.ny : 1;
+            const int64_t band = banded ? (ys + 7) / 8 : ys;
+            for(int64_t y0 = 0; y0 < ys; y0 += band)
+            for(int64_t p0 = 0; p0 < planes; p0 += nwv) {
+                const int nu = (int)std::min<int64_t>(nwv, planes - p0);
+                for(int64_t y = y0; y < std::min<int64_t>(ys, y0 + band); y++)
+                    for(int sg = 0; sg < g.nseg; sg++) {
+                        NbrWork w;
+                        w.inst = i;
+                        w.nunit = (int16_t)nu;
+                        w.seg = (int16_t)sg;
+                        w.y = (int32_t)y;
+                        w.p0 = (int32_t)p0;
+                        nw.push_back(w);
+                    }
+            }
+        }
+        desc[i].np1 = (int32_t)nw.size();
+    }
+    for(int i = 0; i < n_inst; i++) {
+        if(jobs[i]->nbr_mode == 1) {
+            desc[i].np0 += (int32_t)nw.size();
+            desc[i].np1 += (int32_t)nw.size();
+        }
+        jobs[i]->items = (int64_t)(desc[i].lp1 - desc[i].lp0) + (desc[i].np1 - desc[i].np0);
+    }
+    // second-level reduction items over each instance's partial ranges
+    std::vector<RedWork> &lr = t.lr, &nr = t.nr;
+    lr.clear(); nr.clear();
+    for(int i = 0; i < n_inst; i++) {
+        auto split = [&](std::vector<RedWork> &out, int32_t p0, int32_t p1, int32_t &r0, int32_t &r1) {
+            r0 = (int32_t)out.size();
+            for(int32_t p = p0; p < p1; p += REDUCE_SPAN) {
+                RedWork r;
+                r.inst = i;
+                r.p0 = p;
+                r.p1 = std::min<int32_t>(p1, p + REDUCE_SPAN);
+                r.pad = 0;
+                out.push_back(r);
+            }
+            r1 = (int32_t)out.size();
+        };
+        split(lr, desc[i].lp0, desc[i].lp1, desc[i].rl0, desc[i].rl1);
+        split(nr, desc[i].np0, desc[i].np1, desc[i].rn0, desc[i].rn1);
+    }
+}
+
+// The flux-partial layout of a table set, as the kernels use it: local item
+// it writes local slot it; row-segment item it writes neighbour slot it;
+// compact item it writes neighbour slot n_rowseg + it (both partial arrays are
+// allocated with exactly that many slots).  Checks that every slot an item
+// writes lies in its array and in its own instance's range, that the ranges
+// of the instances partition each array, and that each instance's reduction
+// items tile its range in order, each at most REDUCE_SPAN slots, and are the
+// ones its InstDesc names.  Returns the number of violations (the first one
+// described in *first).
+int64_t check_tables(const BatchTables &t, std::string *first)
+{
+    int64_t bad = 0;
+    auto fail = [&](const std::string &m) { if(!bad && first) *first = m; bad++; };
+    const int64_t n_inst = (int64_t)t.desc.size();
+    const int64_t nl = (int64_t)t.lw.size(), nn = (int64_t)t.nw.size() + (int64_t)t.cw.size();
+    auto in_inst = [&](int64_t slot, int32_t inst, bool local) {
+        if(inst < 0 || inst >= n_inst) return false;
+        const InstDesc &D = t.desc[(size_t)inst];
+        return local ? (slot >= D.lp0 && slot < D.lp1) : (slot >= D.np0 && slot < D.np1);
+    };
+    for(int64_t it = 0; it < nl; it++)
+        if(!in_inst(it, t.lw[(size_t)it].inst, true)) fail("local item " + std::to_string(it) + " outside its instance");
+    for(int64_t it = 0; it < (int64_t)t.nw.size(); it++)
+        if(!in_inst(it, t.nw[(size_t)it].inst, false)) fail("row-segment item " + std::to_string(it) + " outside its instance");
+    for(int64_t it = 0; it < (int64_t)t.cw.size(); it++) {
+        const int64_t slot = (int64_t)t.nw.size() + it;
+        if(slot >= nn || !in_inst(slot, t.cw[(size_t)it].inst, false))
+            fail("compact item " + std::to_string(it) + " outside its instance");
+    }
+    for(int local = 0; local < 2; local++) {
+        const int64_t size = local ? nl : nn;
+        const std::vector<RedWork> &rw = local ? t.lr : t.nr;
+        std::vector<int> owner((size_t)size, -1);
+        for(int64_t i = 0; i < n_inst; i++) {
+            const InstDesc &D = t.desc[(size_t)i];
+            const int32_t p0 = local ? D.lp0 : D.np0, p1 = local ? D.lp1 : D.np1;
+            const int32_t r0 = local ? D.rl0 : D.rn0, r1 = local ? D.rl1 : D.rn1;
+            if(p0 < 0 || p1 > size || p0 > p1) { fail("instance " + std::to_string(i) + " range outside the array"); continue; }
+            for(int32_t p = p0; p < p1; p++) {
+                if(owner[(size_t)p] >= 0) fail("slot " + std::to_string(p) + " in two instances");
+                owner[(size_t)p] = (int)i;
+            }
+            int32_t next = p0;
+            if(r0 < 0 || r1 > (int32_t)rw.size() || r0 > r1) { fail("reduction items outside the list"); continue; }
+            for(int32_t r = r0; r < r1; r++) {
+                const RedWork &w = rw[(size_t)r];
+                if(w.inst != i || w.p0 != next || w.p1 <= w.p0 || w.p1 - w.p0 > REDUCE_SPAN)
+                    fail("reduction item " + std::to_string(r) + " does not tile instance " + std::to_string(i));
+                next = w.p1;
+            }
+            if(next != p1) fail("reduction items of instance " + std::to_string(i) + " stop short");
+        }
+        for(int64_t p = 0; p < size; p++)
+            if(owner[(size_t)p] < 0) fail("slot " + std::to_string(p) + " in no instance");
+    }
+    return bad;
+}
+
+// Host only: the tables a batch would build for these structures' forcing
+// runs (3 per 3D structure, 2 per 2D one; neighbour-step forms chosen as the
+// batch chooses them, without building compact tables), checked.
+std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,
+                                const std::vector<int> &dims, int lattice, int prec,
+                                const Tuning &tu, int64_t *bad)
+{
+    std::vector<std::unique_ptr<Geometry>> geos;
+    std::vector<std::unique_ptr<Job>> jobs;
+    std::vector<Job *> ptrs;
+    const int dim = lattice == 3 ? 3 : 2;
+    for(size_t s = 0; s < solids.size(); s++) {
+        std::unique_ptr<Geometry> g(new Geometry);
+        std::vector<SegInfo> h;
+        const int nx = dims[3 * s], ny = dims[3 * s + 1], nz = dim == 3 ? dims[3 * s + 2] : 1;
+        std::string err = build_segments(*g, solids[s], dim, nx, ny, nz, h);
+        if(!err.empty()) return err;
+        g->solid.assign(solids[s], solids[s] + g->nn);
+        g->row_first.resize((size_t)g->nrows + 1);
+        for(int64_t r = 0; r < g->nrows; r++) g->row_first[(size_t)r] = h[(size_t)(r * g->nseg)].start;
+        g->row_first[(size_t)g->nrows] = (uint32_t)g->n_fluid;
+        for(int d = 0; d < dim; d++) {
+            std::unique_ptr<Job> j(new Job);
+            j->geo = g.get();
+            j->lattice = lattice;
+            j->prec = prec;
+            j->stride = pop_stride(g->n_fluid);
+            int m = tu.nbr_mode;
+            if(m == 2) m = (double)g->n_fluid < COMPACT_BELOW_POROSITY * (double)g->nn ? 1 : 0;
+            if(m == 1 && g->nx >= 0xFFFF) m = 0;
+            j->nbr_mode = m;
+            ptrs.push_back(j.get());
+            jobs.push_back(std::move(j));
+        }
+        geos.push_back(std::move(g));
+    }
+    BatchTables t;
+    build_tables(ptrs, lattice, prec, tu, t);
+    std::string first;
+    *bad = check_tables(t, &first);
+    return first;
+}
+
 static std::atomic<int> g_fail_rebuilds{0};
 
 void test_fail_rebuilds(int n) { g_fail_rebuilds.store(n < 0 ? 0 : n); }
@@ -537,121 +764,15 @@ std::string Batch::rebuild()
     b32_ = !(wide && wide[0] == '1');
     for(const Job *j : jobs_)
         if((uint64_t)lattice_q(lattice_) * (uint64_t)j->stride * elem_bytes(prec_) > 0xFFFFFFFFull) b32_ = false;
-    std::vector<InstDesc> desc(n_inst_);
+    BatchTables tab;
+    build_tables(jobs_, lattice_, prec_, tune_, tab);
+    std::vector<InstDesc> &desc = tab.desc;
+    std::vector<LocalWork> &lw = tab.lw, &cw = tab.cw;
+    std::vector<NbrWork> &nw = tab.nw;
+    std::vector<RedWork> &lr = tab.lr, &nr = tab.nr;
     std::vector<InstCfg> cfg(n_inst_);
     std::vector<InstState> state(n_inst_);
-    std::vector<LocalWork> lw, cw;
-    std::vector<NbrWork> nw;
-    for(int i = 0; i < n_inst_; i++) {
-        Job *j = jobs_[i];
-        desc[i] = make_desc(*j);
-        cfg[i] = j->cfg;
-        state[i] = j->state;
-        const Geometry &g = *j->geo;
-        const int64_t lchunk = 256 * (int64_t)tune_.local_cpt;
-        desc[i].lp0 = (int32_t)lw.size();
-        for(int64_t c = 0; c < g.n_fluid; c += lchunk) {
-            LocalWork w;
-            w.inst = i;
-            w.ncell = (int32_t)std::min<int64_t>(lchunk, g.n_fluid - c);
-            w.first = c;
-            lw.push_back(w);
-        }
-        desc[i].lp1 = (int32_t)lw.size();
-        // large lattices deal their neighbour-step items to the XCDs in y bands
-        // (see below)
-        const bool banded = g.dim == 3 &&
-            (double)lattice_q(lattice_) * (double)j->stride * (double)elem_bytes(prec_) >= BAND_MIN_BYTES;
-        if(j->nbr_mode == 1) {
-            // compact neighbour step: up to 256 consecutive fluid cells per
-            // item; its partial slots follow the row-segment ones (offset
-            // fixed below).  Banded: the cells of rows y0..y1-1 of one plane
-            // are consecutive, so band b is plane after plane of such runs
-            // (256^3 FP32 0.385 against 0.406 ms, FP64 0.653 against 0.761;
-            // 512^3 GRF FP32 0.700 against 0.712).
-            desc[i].np0 = (int32_t)cw.size();
-            auto runs = [&](int64_t a0, int64_t a1) {
-                for(int64_t c = a0; c < a1; c += 256) {
-                    LocalWork w;
-                    w.inst = i;
-                    w.ncell = (int32_t)std::min<int64_t>(256, a1 - c);
-                    w.first = c;
-                    cw.push_back(w);
-                }
-            };
-            if(banded) {
-                const int64_t band = (g.ny + 7) / 8;
-                for(int64_t y0 = 0; y0 < g.ny; y0 += band) {
-                    const int64_t y1 = std::min<int64_t>(g.ny, y0 + band);
-                    for(int64_t z = 0; z < g.nz; z++)
-                        runs(g.row_first[(size_t)(z * g.ny + y0)], g.row_first[(size_t)(z * g.ny + y1)]);
-                }
-            } else {
-                runs(0, g.n_fluid);
-            }
-            desc[i].np1 = (int32_t)cw.size();
-            continue;
-        }
-        desc[i].np0 = (int32_t)nw.size();
-        {
-            // slabs of nbr_waves rows stacked along z (2D: y); inside a slab y
-            // then the segment vary fastest.  The kernels split their items
-            // into 8 contiguous ranges, one per XCD.  Large lattices deal the
-            // rows in 8 bands of ny/8 rows, each band through all z, slab after
-            // slab: an XCD then owns one band and walks it in z, so the rows
-            // either side of a slab boundary are in flight together and share
-            // its L2 (256^3 FP32 step 0.318 against 0.332 ms; bands of 8-64
-            // rows inside per-XCD z ranges gave 0.327-0.331).  Below ~400 MB of
-            // populations the plain z order stays: an XCD's neighbour-step
-            // range then covers the same cells as its local-step range, and
-            // the L2 lines the one kernel leaves are the other's (192^3 FP32:
-            // bands 0.139, z order 0.135 ms; 96^3: 0.030 against 0.027).
-            const int64_t nwv = tune_.nbr_waves;
-            const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
-            const int64_t ys = g.dim == 3 ? g.ny : 1;
-            const int64_t band = banded ? (ys + 7) / 8 : ys;
-            for(int64_t y0 = 0; y0 < ys; y0 += band)
-            for(int64_t p0 = 0; p0 < planes; p0 += nwv) {
-                const int nu = (int)std::min<int64_t>(nwv, planes - p0);
-                for(int64_t y = y0; y < std::min<int64_t>(ys, y0 + band); y++)
-                    for(int sg = 0; sg < g.nseg; sg++) {
-                        NbrWork w;
-                        w.inst = i;
-                        w.nunit = (int16_t)nu;
-                        w.seg = (int16_t)sg;
-                        w.y = (int32_t)y;
-                        w.p0 = (int32_t)p0;
-                        nw.push_back(w);
-                    }
-            }
-        }
-        desc[i].np1 = (int32_t)nw.size();
-    }
-    for(int i = 0; i < n_inst_; i++) {
-        if(jobs_[i]->nbr_mode == 1) {
-            desc[i].np0 += (int32_t)nw.size();
-            desc[i].np1 += (int32_t)nw.size();
-        }
-        jobs_[i]->items = (int64_t)(desc[i].lp1 - desc[i].lp0) + (desc[i].np1 - desc[i].np0);
-    }
-    // second-level reduction items over each instance's partial ranges
-    std::vector<RedWork> lr, nr;
-    for(int i = 0; i < n_inst_; i++) {
-        auto split = [&](std::vector<RedWork> &out, int32_t p0, int32_t p1, int32_t &r0, int32_t &r1) {
-            r0 = (int32_t)out.size();
-            for(int32_t p = p0; p < p1; p += REDUCE_SPAN) {
-                RedWork r;
-                r.inst = i;
-                r.p0 = p;
-                r.p1 = std::min<int32_t>(p1, p + REDUCE_SPAN);
-                r.pad = 0;
-                out.push_back(r);
-            }
-            r1 = (int32_t)out.size();
-        };
-        split(lr, desc[i].lp0, desc[i].lp1, desc[i].rl0, desc[i].rl1);
-        split(nr, desc[i].np0, desc[i].np1, desc[i].rn0, desc[i].rn1);
-    }
+    for(int i = 0; i < n_inst_; i++) { cfg[i] = jobs_[i]->cfg; state[i] = jobs_[i]->state; }
     n_lred_ = (int)lr.size();
     n_nred_ = (int)nr.size();
     if(lw.size() > INT32_MAX || nw.size() > INT32_MAX || cw.size() > INT32_MAX) return "batch too large";

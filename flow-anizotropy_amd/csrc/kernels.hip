@@ -703,8 +703,11 @@ static void launch_step(bool local, const Tuning &tu, bool b32, const InstDesc *
     }
 }
 
-// plain device copy, the bandwidth reference for the step kernels
-// 4 loads in flight per thread; a workgroup moves 16 KiB per iteration
+// plain device copies, the bandwidth reference for the step kernels.
+// k_copy: a persistent grid, 4 loads in flight per thread, 16 KiB per
+// workgroup and iteration.  k_copy_flat: one workgroup per 8 KiB, two float4
+// per thread, both loads issued before the stores; NT: loads and stores that
+// do not allocate in the caches (the form of the local step's streams).
 __global__ void __launch_bounds__(256)
 k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n)
 {
@@ -724,10 +727,31 @@ k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n)
     }
 }
 
-void kern_copy(const void *a, void *b, int64_t n16, hipStream_t s)
+template<bool NT>
+__global__ void __launch_bounds__(256)
+k_copy_flat(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n)
 {
-    const int64_t blocks = std::min<int64_t>((n16 + 1023) / 1024, 256 * 32);
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)a, (float4 *)b, n16);
+    using V4 = VecT<float, 4>::type;
+    gptr<V4> src = (gptr<V4>)(void *)a;
+    gptr<V4> dst = (gptr<V4>)(void *)b;
+    const int64_t i0 = (int64_t)blockIdx.x * 512 + threadIdx.x, i1 = i0 + 256;
+    V4 v0, v1;
+    if(i0 < n) v0 = NT ? __builtin_nontemporal_load(src + i0) : src[i0];
+    if(i1 < n) v1 = NT ? __builtin_nontemporal_load(src + i1) : src[i1];
+    if(i0 < n) { if constexpr(NT) __builtin_nontemporal_store(v0, dst + i0); else dst[i0] = v0; }
+    if(i1 < n) { if constexpr(NT) __builtin_nontemporal_store(v1, dst + i1); else dst[i1] = v1; }
+}
+
+void kern_copy(int kind, const void *a, void *b, int64_t n16, hipStream_t s)
+{
+    if(kind == 0) {
+        const int64_t blocks = std::min<int64_t>((n16 + 1023) / 1024, 256 * 32);
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)a, (float4 *)b, n16);
+        return;
+    }
+    const dim3 g((unsigned)((n16 + 511) / 512)), t(256);
+    if(kind == 1) hipLaunchKernelGGL((k_copy_flat<false>), g, t, 0, s, (const float4 *)a, (float4 *)b, n16);
+    else          hipLaunchKernelGGL((k_copy_flat<true>), g, t, 0, s, (const float4 *)a, (float4 *)b, n16);
 }
 
 void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s)

@@ -293,6 +293,18 @@ typedef struct fp_tuning {
 int  fp_set_tuning(fp_ctx *ctx, const fp_tuning *t);
 void fp_get_tuning(fp_ctx *ctx, fp_tuning *t);
 
+/* Self-check of a batch's flux-partial layout (no reference counterpart; host
+ * only, no device work): builds the tables a batch would for the forcing runs
+ * of n structures (3 per 3D mask, 2 per 2D one; dims[3i..3i+2] = nx, ny, nz of
+ * mask i, nz ignored in 2D), with the neighbour-step forms chosen as the batch
+ * chooses them under tune (NULL: the defaults), and checks that every
+ * flux-partial slot a step kernel writes lies in its array and in its own
+ * instance's range, that the instances' ranges partition each array and that
+ * each instance's reduction items tile its range.  *problems = 0 when all
+ * hold. */
+int fp_verify_batch_tables(const uint8_t *const *solids, const int32_t *dims, int n, int lattice,
+                           int precision, const fp_tuning *tune, int64_t *problems);
+
 /* Steps one resident instance: `warmup` then `steps` steps launched one by one
  * with events around every kernel (the per-kernel averages), then `steps` more
  * as hipGraph chunk replays (total_ms).  tune: NULL = the context's shapes.
@@ -398,8 +410,10 @@ void       fp_server_get_stats(fp_server *s, fp_server_stats *out);
  * would, before they touch the device.  0 clears it. */
 void fp_test_fail_rebuilds(int n);
 
-/* Plain device-to-device copy of `bytes` (read + write), `iters` times:
- * the measured bandwidth the step kernels are compared with. */
+/* Plain device-to-device copy of `bytes` (read + write), `iters` times with
+ * each of three copy kernels (a persistent grid; one workgroup per 8 KiB; the
+ * same with non-temporal loads and stores): *ms_per_copy is the fastest, the
+ * measured bandwidth the step kernels are compared with. */
 int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_per_copy);
 
 #ifdef __cplusplus

@@ -18,6 +18,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+# the flux normalisation length of the reference's 256^2 lattice (lbm.h:24-27);
+# the Python surface would scale it with the lattice, the oracle takes it as given
+L0 = 4.0
+
+
 def slit2d(L=32, h=6):
     m = np.zeros((L, L), np.uint8)
     m[:L - h, :] = 1
@@ -37,8 +42,8 @@ def test_slit_stops_at_the_minimum_at_reference_defaults(gpu_ctx, oracle, prec, 
     from flowperm import solver
     m = slit2d()
     print(f"2D slit at the reference defaults, {prec}", flush=True)
-    r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision=prec)
-    rc, T = oracle.solve_tensor2d(m, double=double)
+    r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision=prec, L0=L0)
+    rc, T = oracle.solve_tensor2d(m, double=double, L0=L0)
     assert r["exit_code"] == rc == 0
     for t in ("fx", "fy"):
         assert r[f"steps_{t}"] == 40000 and r[f"conv_{t}"] == 1 and r[f"fieldres_{t}"] == 0.0
@@ -49,9 +54,9 @@ def test_direction_without_a_through_path_converges_to_the_closed_form(gpu_ctx, 
     from flowperm import solver
     m = np.ones((32, 32), np.uint8)
     m[:, 10:16] = 0
-    r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision="f64")
-    rc, T = oracle.solve_tensor2d(m, double=True)
-    want = -2.5e-7 * int((m == 0).sum()) / (2 * 32 * 4.0 ** 2)
+    r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision="f64", L0=L0)
+    rc, T = oracle.solve_tensor2d(m, double=True, L0=L0)
+    want = -2.5e-7 * int((m == 0).sum()) / (2 * 32 * L0 ** 2)
     assert r["steps_fx"] == 40000 and r["conv_fx"] == 1
     assert abs(r["qx_fx"] - want) <= 1e-9 * abs(want)
     _check_against_oracle(r, T)
@@ -61,8 +66,8 @@ def test_direction_without_a_through_path_converges_to_the_closed_form(gpu_ctx, 
 def test_max_steps_below_the_minimum_is_exit_3(gpu_ctx, oracle, prec, double):
     from flowperm import solver
     m = slit2d()
-    r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision=prec, max_steps=15000)
-    rc, T = oracle.solve_tensor2d(m, double=double, max_steps=15000)
+    r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision=prec, max_steps=15000, L0=L0)
+    rc, T = oracle.solve_tensor2d(m, double=double, max_steps=15000, L0=L0)
     assert r["exit_code"] == rc == 3
     for t in ("fx", "fy"):
         assert r[f"steps_{t}"] == 35000 and r[f"conv_{t}"] == 0
@@ -73,10 +78,11 @@ def test_max_steps_below_the_minimum_is_exit_3(gpu_ctx, oracle, prec, double):
 @pytest.mark.parametrize("avg,want", [(100, 1.0), (0, 10.0)])
 def test_no_first_phase_snapshots_the_rest_field(gpu_ctx, oracle, prec, avg, want):
     """fieldres from the rest-state snapshot (resetdistributions zeroes U, V,
-    LMB2d/lbm.cpp:200-211); earlier the snapshot slot was never written here."""
+    LMB2d/lbm.cpp:200-211): the snapshot slot must hold the rest field when no
+    first-phase step writes it."""
     from flowperm import solver
     m = slit2d()
-    cfg = dict(max_steps=0, avg_steps=avg, avg_sample=25)
+    cfg = dict(max_steps=0, avg_steps=avg, avg_sample=25, L0=L0)
     r = solver.solve_permeability2d(m, ctx=gpu_ctx, precision=prec, **cfg)
     for t in ("fx", "fy"):
         assert r[f"fieldres_{t}"] == want and r[f"steps_{t}"] == avg and r[f"conv_{t}"] == 0
