@@ -386,6 +386,9 @@ fp_ctx *fp_create(int device, int *status)
         delete c;
         return nullptr;
     }
+    // the address of fp_solve_dir3d_noalloc's scratch array on this device
+    // (a failure is reported by that entry point)
+    (void)noalloc_setup(device);
     if(status) *status = FP_OK;
     return c;
 }

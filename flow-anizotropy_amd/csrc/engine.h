@@ -46,6 +46,13 @@ struct Geometry {
     size_t compact_bytes() const;
 };
 
+// The host half of geometry_build: the segment table, the link records
+// (LinkRec<Q-1> per segment, as bytes) and the wrap table of a mask, plus the
+// Geometry's sizes, solid copy and row starts; no device memory is touched
+// (g's device pointers stay null).
+std::string geometry_host_tables(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz,
+                                 std::vector<SegInfo> &seg, std::vector<unsigned char> &links,
+                                 std::vector<uint32_t> &rowspec);
 // builds the segment table from a mask (x fastest; 1 = solid) and uploads it.
 // Returns an empty string or the error.
 std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz);
@@ -75,6 +82,11 @@ struct Job {
     void *user = nullptr;
     size_t device_bytes() const;
 };
+
+// cells a job's population arrays hold (n_fluid rounded up to 1024)
+int64_t pop_stride_of(int64_t n_fluid);
+// the device descriptor of a job (its geometry's and its own device pointers)
+InstDesc job_desc(const Job &j);
 
 // bytes of device memory a job of this shape needs
 size_t job_bytes(const Geometry &g, int lattice, int prec, bool fields);

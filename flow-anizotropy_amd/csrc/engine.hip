@@ -198,6 +198,30 @@ static std::string build_segments(Geometry &g, const uint8_t *solid, int dim, in
     return "";
 }
 
+std::string geometry_host_tables(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz,
+                                 std::vector<SegInfo> &seg, std::vector<unsigned char> &links,
+                                 std::vector<uint32_t> &rowspec)
+{
+    std::string err = build_segments(g, solid, dim, nx, ny, nz, seg);
+    if(!err.empty()) return err;
+    if(dim == 3) {
+        std::vector<LinkRec<D3Q19::Q - 1>> l;
+        build_links<D3Q19>(g, seg, l, rowspec);
+        links.assign((const unsigned char *)l.data(), (const unsigned char *)(l.data() + l.size()));
+    } else {
+        std::vector<LinkRec<D2Q9::Q - 1>> l;
+        build_links<D2Q9>(g, seg, l, rowspec);
+        links.assign((const unsigned char *)l.data(), (const unsigned char *)(l.data() + l.size()));
+    }
+    g.links_bytes = links.size();
+    g.rowspec_bytes = rowspec.size() * sizeof(uint32_t);
+    g.solid.assign(solid, solid + g.nn);
+    g.row_first.resize((size_t)g.nrows + 1);
+    for(int64_t r = 0; r < g.nrows; r++) g.row_first[(size_t)r] = seg[(size_t)(r * g.nseg)].start;
+    g.row_first[(size_t)g.nrows] = (uint32_t)g.n_fluid;
+    return "";
+}
+
 std::string geometry_build(Geometry &g, const uint8_t *solid, int dim, int nx, int ny, int nz)
 {
     std::vector<SegInfo> h;
@@ -405,6 +429,9 @@ static InstDesc make_desc(const Job &j)
     if(j.lattice == 3) f32_force_terms(j.F, D.g45, D.h15);
     return D;
 }
+
+int64_t pop_stride_of(int64_t n_fluid) { return pop_stride(n_fluid); }
+InstDesc job_desc(const Job &j) { return make_desc(j); }
 
 // ---------------------------------------------------------------------------
 

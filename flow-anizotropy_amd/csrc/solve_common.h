@@ -41,5 +41,8 @@ void assemble2d(fp_result2d *out, int lx, int ly, const uint8_t *solid, double F
 void validate2d_force(double F, double deg, double Fv[2]);
 void assemble2d_val(fp_result2d *out, double F, double deg);
 std::string finish2d(const Job &j, int lx, int ly, int64_t avg_steps, fp_solve_out *o, float *vel);
+// the allocation-free solve's scratch array: look up its address on `device`
+// (the current device), once (noalloc.hip)
+hipError_t noalloc_setup(int device);
 
 } // namespace fp

@@ -172,6 +172,8 @@ SIGNATURES = {
     "fp_replay_convergence2d": (C.c_int, [C.POINTER(Config2D), _dp, C.c_int64, C.POINTER(SolveOut)]),
     "fp_field_residual2d": (C.c_double, [C.c_int, C.c_int, _fp, _fp]),
     "fp_test_fail_rebuilds": (None, [C.c_int]),
+    "fp_solve_dir3d_noalloc": (C.c_int, [C.c_void_p, _u8p, C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(Config3D), _dp, C.POINTER(SolveOut)]),
     "fp_verify_batch_tables": (C.c_int, [C.POINTER(_u8p), C.POINTER(C.c_int32), C.c_int, C.c_int,
                                          C.c_int, C.POINTER(Tuning), C.POINTER(C.c_int64)]),
 }

@@ -142,6 +142,22 @@ int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
 int fp_solve_dir3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
                    const fp_config3d *cfg, const double F[3], fp_solve_out *out, float *vel);
 
+/* fp_solve_dir3d without device allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code that must not call
+ * hipMalloc, tight serving loops, harnesses that forbid allocation).  The same
+ * forced solve, bit for bit, with no velocity export: every device buffer of
+ * it -- tables, populations, state, work lists, flux partials -- lies in one
+ * array of FP_NOALLOC_SCRATCH_BYTES that the library holds on each device it
+ * runs on until the process ends, and no hipMalloc, hipFree or allocating
+ * library call is made.  A lattice whose buffers exceed the array is rejected
+ * with FP_ERR_ARG (the bytes it needs in fp_last_error): 64^3 in FP64 and
+ * 80^3 in FP32 fit at any porosity.  Every call on a device uses that device's
+ * one array and leaves its contents behind; calls on one device are serialised
+ * by the library (two at once would race on the array). */
+#define FP_NOALLOC_SCRATCH_BYTES 134217728
+int fp_solve_dir3d_noalloc(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
+                           const fp_config3d *cfg, const double F[3], fp_solve_out *out);
+
 /* Self-check of the neighbour tables the step kernels use (no reference
  * counterpart; host only, no device work): for every fluid cell and lattice
  * direction, the neighbour index the kernels derive from the per-segment link
