@@ -11,9 +11,10 @@ namespace fp {
 
 // kernels.hip
 void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
-// b32: every instance's population arrays span less than 4 GiB (32-bit byte
-// offsets in the neighbour kernels)
-void kern_step(int lattice, int prec, bool local, const Tuning &tu, bool b32, const InstDesc *insts,
+// ad: how the step kernels address populations (Addr: AD_B32 32-bit byte
+// offsets, every instance spans less than 4 GiB; AD_E32 32-bit element
+// indices; AD_E64 64-bit element indices, instances of 2^32 elements or more)
+void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, const InstDesc *insts,
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
 // plain copies of n16 float4: kind 0 persistent grid, 1 one workgroup per
 // 8 KiB, 2 the same with non-temporal loads and stores
@@ -171,7 +172,7 @@ private:
     LocalWork *d_cwork_ = nullptr;           // compact neighbour-step items (mode 1)
     size_t cap_cwork_ = 0;
     int mode_ = 0;                           // forms in the tables: 0, 1 or 2 (both)
-    bool b32_ = true;                        // every instance's populations span < 4 GiB
+    int ad_ = AD_B32;                        // population addressing of the step kernels (Addr)
     int n_cwork_ = 0;
     Partial *d_lpart_ = nullptr, *d_npart_ = nullptr;
     RedWork *d_lredw_ = nullptr, *d_nredw_ = nullptr;     // second-level reduction items

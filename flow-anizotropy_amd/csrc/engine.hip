@@ -483,8 +483,6 @@ std::string Batch::admit(Job *j)
     if(j->lattice != lattice_ || j->prec != prec_) return "job does not match the batch type";
     const Geometry &g = *j->geo;
     j->stride = pop_stride(g.n_fluid);
-    if((uint64_t)lattice_q(lattice_) * (uint64_t)j->stride >= (1ull << 32))
-        return "lattice too large for 32-bit population offsets";
     const size_t fb = (size_t)lattice_q(lattice_) * j->stride * elem_bytes(prec_);
     hipError_t e = hipMalloc(&j->d_f, fb);
     if(e != hipSuccess) { j->d_f = nullptr; (void)hipGetLastError(); return "out of device memory"; }
@@ -785,12 +783,20 @@ std::string Batch::rebuild()
         n_mode[m]++;
     }
     mode_ = n_mode[1] == 0 ? 0 : (n_mode[0] == 0 ? 1 : 2);
-    // FLOWPERM_WIDE_OFFSETS=1 forces the 64-bit addressing kernels (the ones
-    // instances of 4 GiB and more use), so tests can compare the two
+    // the narrowest population addressing that covers every instance:
+    // 32-bit byte offsets below 4 GiB, 32-bit element indices below 2^32
+    // elements, 64-bit element indices beyond.  FLOWPERM_WIDE_OFFSETS=1 or 2
+    // forces at least the element-index or the 64-bit kernels, so tests can
+    // compare them with the narrow ones on small lattices.
     const char *wide = getenv("FLOWPERM_WIDE_OFFSETS");
-    b32_ = !(wide && wide[0] == '1');
-    for(const Job *j : jobs_)
-        if((uint64_t)lattice_q(lattice_) * (uint64_t)j->stride * elem_bytes(prec_) > 0xFFFFFFFFull) b32_ = false;
+    ad_ = AD_B32;
+    if(wide && wide[0] == '1') ad_ = AD_E32;
+    if(wide && wide[0] == '2') ad_ = AD_E64;
+    for(const Job *j : jobs_) {
+        const uint64_t elems = (uint64_t)lattice_q(lattice_) * (uint64_t)j->stride;
+        if(elems * elem_bytes(prec_) > 0xFFFFFFFFull) ad_ = std::max(ad_, (int)AD_E32);
+        if(elems > 0xFFFFFFFFull) ad_ = AD_E64;
+    }
     BatchTables tab;
     build_tables(jobs_, lattice_, prec_, tune_, tab);
     std::vector<InstDesc> &desc = tab.desc;
@@ -927,18 +933,18 @@ void Batch::launch_finish(bool local)
 void Batch::launch_step(bool local)
 {
     if(local) {
-        kern_step(lattice_, prec_, true, tune_, b32_, d_inst_, d_state_, d_lwork_, n_lwork_, d_lpart_, stream_);
+        kern_step(lattice_, prec_, true, tune_, ad_, d_inst_, d_state_, d_lwork_, n_lwork_, d_lpart_, stream_);
         return;
     }
     // the row-segment items, then the compact ones (their partial slots follow)
     Tuning tu = tune_;
     if(n_nwork_ > 0) {
         tu.nbr_mode = 0;
-        kern_step(lattice_, prec_, false, tu, b32_, d_inst_, d_state_, d_nwork_, n_nwork_, d_npart_, stream_);
+        kern_step(lattice_, prec_, false, tu, ad_, d_inst_, d_state_, d_nwork_, n_nwork_, d_npart_, stream_);
     }
     if(n_cwork_ > 0) {
         tu.nbr_mode = 1;
-        kern_step(lattice_, prec_, false, tu, b32_, d_inst_, d_state_, d_cwork_, n_cwork_,
+        kern_step(lattice_, prec_, false, tu, ad_, d_inst_, d_state_, d_cwork_, n_cwork_,
                   d_npart_ + n_nwork_, stream_);
     }
 }

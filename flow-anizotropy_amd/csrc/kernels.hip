@@ -51,15 +51,19 @@ template<bool NT, class V> __device__ __forceinline__ void stv(gptr<V> p, V v)
 }
 template<class T> __device__ __forceinline__ cptr<T> as_const(const void *p) { return (cptr<T>)p; }
 
-// population element `off` of an instance.  B32: the instance's arrays span
-// less than 4 GiB, so the byte offset is a 32-bit lane value added to the
-// wave-uniform base (the `global_load v, v_off, s[base]` form: one VGPR per
-// address instead of two).  The neighbour kernels hold 19 addresses at once;
-// at 60 instead of 68 VGPRs eight wavefronts fit per SIMD instead of seven and
-// the FP32 row-segment step takes 0.400 instead of 0.440 ms at 256^3.
-template<bool B32, class T> __device__ __forceinline__ gptr<T> pop_ptr(gptr<T> f, uint32_t off)
+// population element `off` of an instance, addressed in mode AD (Addr).
+// AD_B32: the instance's arrays span less than 4 GiB, so the byte offset is a
+// 32-bit lane value added to the wave-uniform base (the `global_load v, v_off,
+// s[base]` form: one VGPR per address instead of two).  The neighbour kernels
+// hold 19 addresses at once; at 60 instead of 68 VGPRs eight wavefronts fit per
+// SIMD instead of seven and the FP32 row-segment step takes 0.400 instead of
+// 0.440 ms at 256^3.  AD_E32: a 32-bit element index; AD_E64: a 64-bit one (an
+// instance of 2^32 population elements or more, 226 M fluid cells in D3Q19).
+template<int AD> struct OffOf { using T = uint32_t; };
+template<> struct OffOf<AD_E64> { using T = uint64_t; };
+template<int AD, class T> __device__ __forceinline__ gptr<T> pop_ptr(gptr<T> f, typename OffOf<AD>::T off)
 {
-    if constexpr(B32) return (gptr<T>)((gptr<char>)f + off * (uint32_t)sizeof(T));
+    if constexpr(AD == AD_B32) return (gptr<T>)((gptr<char>)f + off * (uint32_t)sizeof(T));
     else return f + off;
 }
 
@@ -218,7 +222,7 @@ k_rest(InstDesc D)
         else if constexpr(L::DIM == 3) w = (T)L::w(k);
         else if constexpr(P == PREC_F64) w = (T)(k == 0 ? 4.0/9.0 : (k < 5 ? 1.0/9.0 : 1.0/36.0));
         else w = (T)L::wf(k);
-        f[pop_at<L::Q>(k, (uint32_t)i)] = w;
+        f[pop_at64<L::Q>(k, (uint64_t)i)] = w;
     }
     if(i < D.n_fluid) {
 #pragma unroll
@@ -261,11 +265,16 @@ template<class T, int N> __device__ __forceinline__ T &lane_of(typename VecT<T, 
     else return reinterpret_cast<T *>(&v)[c];
 }
 
-template<class L, int P, int CPT, bool NT>
+template<class L, int P, int CPT, bool NT, bool W>
 __global__ void __launch_bounds__(256)
 k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part)
 {
+    // W: 64-bit element indices (arrays of 2^32 elements or more)
+    auto at = [](int k, int64_t i) -> uint64_t {
+        if constexpr(W) return pop_at64<L::Q>(k, (uint64_t)i);
+        else return pop_at<L::Q>(k, (uint32_t)i);
+    };
     using T = typename Store<L, P>::T;
     using V = typename VecT<T, CPT>::type;
     const ItemRange R = my_items(nwork);
@@ -283,7 +292,7 @@ k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         if(nreal > 0) {
             V in[L::Q];
 #pragma unroll
-            for(int k = 0; k < L::Q; k++) in[k] = ld<NT>((gptr<V>)(f + pop_at<L::Q>(k, (uint32_t)i)));
+            for(int k = 0; k < L::Q; k++) in[k] = ld<NT>((gptr<V>)(f + at(k, i)));
             V out[L::Q];
 #pragma unroll
             for(int c = 0; c < CPT; c++) {
@@ -297,7 +306,7 @@ k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
                     lane_of<T, CPT>(out[L::opp(k)], c) = real ? v[k] : lane_of<T, CPT>(in[L::opp(k)], c);
             }
 #pragma unroll
-            for(int k = 0; k < L::Q; k++) stv<NT>((gptr<V>)(f + pop_at<L::Q>(k, (uint32_t)i)), out[k]);
+            for(int k = 0; k < L::Q; k++) stv<NT>((gptr<V>)(f + at(k, i)), out[k]);
         }
         if(vslot && nreal > 0) {
 #pragma unroll
@@ -353,7 +362,7 @@ __device__ __forceinline__ uint32_t lane_select(uint64_t M, uint32_t a, uint32_t
     return r;
 }
 
-template<class L, int P, bool B32>
+template<class L, int P, int AD>
 __global__ void __launch_bounds__(512)
 k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
       const NbrWork *__restrict__ work, int nwork, Partial *__restrict__ part)
@@ -404,21 +413,25 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
                 Bk[k] = lr->base[k];
                 Sk[k] = L::ex(k + 1) != 0 ? rs[k] : 0u;
             }
-            uint32_t off[Q];
-            off[0] = pop_at<Q>(0, i);
+            using O = typename OffOf<AD>::T;
+            O off[Q];
+            off[0] = pop_at_as<Q, O>(0, i);
 #pragma unroll
             for(int k = 1; k < Q; k++) {
                 uint32_t nbi = lane_rank_plus(Mk[k - 1], Bk[k - 1]);
                 if(L::ex(k) > 0) nbi = wrapL ? Sk[k - 1] : nbi;
                 if(L::ex(k) < 0) nbi = wrapR ? Sk[k - 1] : nbi;
-                off[k] = lane_select(Mk[k - 1], pop_at<Q>(L::opp(k), nbi), pop_at<Q>(k, i));
+                if constexpr(AD == AD_E64)
+                    off[k] = ((Mk[k - 1] >> lane) & 1ull) ? pop_at64<Q>(L::opp(k), nbi) : pop_at64<Q>(k, i);
+                else
+                    off[k] = lane_select(Mk[k - 1], pop_at<Q>(L::opp(k), nbi), pop_at<Q>(k, i));
             }
             T v[Q];
 #pragma unroll
-            for(int k = 0; k < Q; k++) v[k] = *pop_ptr<B32>(f, off[k]);
+            for(int k = 0; k < Q; k++) v[k] = *pop_ptr<AD>(f, off[k]);
             node<L, P>(v, D, u);
 #pragma unroll
-            for(int k = 0; k < Q; k++) *pop_ptr<B32>(f, off[L::opp(k)]) = v[k];
+            for(int k = 0; k < Q; k++) *pop_ptr<AD>(f, off[L::opp(k)]) = v[k];
             if(vslot) write_vel<L>(D, vslot, (int64_t)i, u);
         }
         if(flux) {
@@ -440,7 +453,7 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 // case), else from vector loads.  The same slots as k_nbr are read and
 // written, so the two forms give identical results.
 // ---------------------------------------------------------------------------
-template<class L, int P, bool B32>
+template<class L, int P, int AD>
 __global__ void __launch_bounds__(256)
 k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part)
@@ -500,14 +513,15 @@ k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
             rk[0] = i - start[0];
 #pragma unroll
             for(int m = 1; m < NR; m++) rk[m] = r.c.rank[m - 1];
-            uint32_t off[Q];
-            compact_offsets<L>(i, r.c.flags, rk, start, lastr, off);
+            using O = typename OffOf<AD>::T;
+            O off[Q];
+            compact_offsets<L, O>(i, r.c.flags, rk, start, lastr, off);
             T v[Q];
 #pragma unroll
-            for(int k = 0; k < Q; k++) v[k] = *pop_ptr<B32>(f, off[k]);
+            for(int k = 0; k < Q; k++) v[k] = *pop_ptr<AD>(f, off[k]);
             node<L, P>(v, D, u);
 #pragma unroll
-            for(int k = 0; k < Q; k++) *pop_ptr<B32>(f, off[L::opp(k)]) = v[k];
+            for(int k = 0; k < Q; k++) *pop_ptr<AD>(f, off[L::opp(k)]) = v[k];
             if(vslot) write_vel<L>(D, vslot, (int64_t)i, u);
         }
         if(flux) {
@@ -662,44 +676,51 @@ static int grid_for(int nwork, int blocks_per_cu)
     return (int)std::max<int64_t>(std::min<int64_t>(nwork, g), std::min(nwork, 8));
 }
 
-template<class L, int P>
-static void launch_step(bool local, const Tuning &tu, bool b32, const InstDesc *insts, const InstState *st,
-                        const void *work, int nwork0, Partial *part, hipStream_t s)
+template<class L, int P, int CPT, bool NT>
+static void launch_local(const Tuning &tu, bool wide, const InstDesc *insts, const InstState *st,
+                         const LocalWork *lw, int nwork, Partial *part, hipStream_t s)
 {
-    const int nwork = nwork0;
+    const dim3 g(grid_for(nwork, tu.local_bpc)), b(256);
+    if(wide) hipLaunchKernelGGL((k_local<L, P, CPT, NT, true>), g, b, 0, s, insts, st, lw, nwork, part);
+    else     hipLaunchKernelGGL((k_local<L, P, CPT, NT, false>), g, b, 0, s, insts, st, lw, nwork, part);
+}
+
+template<class L, int P, int AD>
+static void launch_nbr(const Tuning &tu, const InstDesc *insts, const InstState *st, const void *work,
+                       int nwork, Partial *part, hipStream_t s)
+{
+    // the neighbour kernel's scattered partial-line traffic needs the L2:
+    // it never uses non-temporal accesses (measured 1.6-1.8x the bytes)
+    if(tu.nbr_mode == 1) {
+        const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(256);
+        hipLaunchKernelGGL((k_nbr_c<L, P, AD>), g, b, 0, s, insts, st, (const LocalWork *)work, nwork, part);
+    } else {
+        const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
+        hipLaunchKernelGGL((k_nbr<L, P, AD>), g, b, 0, s, insts, st, (const NbrWork *)work, nwork, part);
+    }
+}
+
+template<class L, int P>
+static void launch_step(bool local, const Tuning &tu, int ad, const InstDesc *insts, const InstState *st,
+                        const void *work, int nwork, Partial *part, hipStream_t s)
+{
     if(nwork <= 0) return;
     if(local) {
         const LocalWork *lw = (const LocalWork *)work;
-        const dim3 g(grid_for(nwork, tu.local_bpc)), b(256);
+        const bool wide = ad == AD_E64;
         if(tu.nontemporal) {
-            if(tu.local_cpt == 4)
-                hipLaunchKernelGGL((k_local<L, P, 4, true>), g, b, 0, s, insts, st, lw, nwork, part);
-            else if(tu.local_cpt == 2)
-                hipLaunchKernelGGL((k_local<L, P, 2, true>), g, b, 0, s, insts, st, lw, nwork, part);
-            else
-                hipLaunchKernelGGL((k_local<L, P, 1, true>), g, b, 0, s, insts, st, lw, nwork, part);
+            if(tu.local_cpt == 4)      launch_local<L, P, 4, true>(tu, wide, insts, st, lw, nwork, part, s);
+            else if(tu.local_cpt == 2) launch_local<L, P, 2, true>(tu, wide, insts, st, lw, nwork, part, s);
+            else                       launch_local<L, P, 1, true>(tu, wide, insts, st, lw, nwork, part, s);
         } else {
-            if(tu.local_cpt == 4)
-                hipLaunchKernelGGL((k_local<L, P, 4, false>), g, b, 0, s, insts, st, lw, nwork, part);
-            else if(tu.local_cpt == 2)
-                hipLaunchKernelGGL((k_local<L, P, 2, false>), g, b, 0, s, insts, st, lw, nwork, part);
-            else
-                hipLaunchKernelGGL((k_local<L, P, 1, false>), g, b, 0, s, insts, st, lw, nwork, part);
+            if(tu.local_cpt == 4)      launch_local<L, P, 4, false>(tu, wide, insts, st, lw, nwork, part, s);
+            else if(tu.local_cpt == 2) launch_local<L, P, 2, false>(tu, wide, insts, st, lw, nwork, part, s);
+            else                       launch_local<L, P, 1, false>(tu, wide, insts, st, lw, nwork, part, s);
         }
     } else {
-        // the neighbour kernel's scattered partial-line traffic needs the L2:
-        // it never uses non-temporal accesses (measured 1.6-1.8x the bytes)
-        if(tu.nbr_mode == 1) {
-            const LocalWork *cw = (const LocalWork *)work;
-            const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(256);
-            if(b32) hipLaunchKernelGGL((k_nbr_c<L, P, true>), g, b, 0, s, insts, st, cw, nwork, part);
-            else    hipLaunchKernelGGL((k_nbr_c<L, P, false>), g, b, 0, s, insts, st, cw, nwork, part);
-        } else {
-            const NbrWork *nw = (const NbrWork *)work;
-            const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
-            if(b32) hipLaunchKernelGGL((k_nbr<L, P, true>), g, b, 0, s, insts, st, nw, nwork, part);
-            else    hipLaunchKernelGGL((k_nbr<L, P, false>), g, b, 0, s, insts, st, nw, nwork, part);
-        }
+        if(ad == AD_B32)      launch_nbr<L, P, AD_B32>(tu, insts, st, work, nwork, part, s);
+        else if(ad == AD_E32) launch_nbr<L, P, AD_E32>(tu, insts, st, work, nwork, part, s);
+        else                  launch_nbr<L, P, AD_E64>(tu, insts, st, work, nwork, part, s);
     }
 }
 
@@ -766,16 +787,16 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s)
     }
 }
 
-void kern_step(int lattice, int prec, bool local, const Tuning &tu, bool b32, const InstDesc *insts,
+void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, const InstDesc *insts,
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s)
 {
     if(lattice == 3) {
-        if(prec == PREC_F64) launch_step<D3Q19, PREC_F64>(local, tu, b32, insts, st, work, nwork, part, s);
-        else                 launch_step<D3Q19, PREC_F32>(local, tu, b32, insts, st, work, nwork, part, s);
+        if(prec == PREC_F64) launch_step<D3Q19, PREC_F64>(local, tu, ad, insts, st, work, nwork, part, s);
+        else                 launch_step<D3Q19, PREC_F32>(local, tu, ad, insts, st, work, nwork, part, s);
     } else {
-        if(prec == PREC_F64)      launch_step<D2Q9, PREC_F64>(local, tu, b32, insts, st, work, nwork, part, s);
-        else if(prec == PREC_F32) launch_step<D2Q9, PREC_F32>(local, tu, b32, insts, st, work, nwork, part, s);
-        else                      launch_step<D2Q9, PREC_F32_REF2D>(local, tu, b32, insts, st, work, nwork, part, s);
+        if(prec == PREC_F64)      launch_step<D2Q9, PREC_F64>(local, tu, ad, insts, st, work, nwork, part, s);
+        else if(prec == PREC_F32) launch_step<D2Q9, PREC_F32>(local, tu, ad, insts, st, work, nwork, part, s);
+        else                      launch_step<D2Q9, PREC_F32_REF2D>(local, tu, ad, insts, st, work, nwork, part, s);
     }
 }
 

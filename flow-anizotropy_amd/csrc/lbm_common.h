@@ -161,11 +161,24 @@ FP_HD void fix_split(double u, double &c0, double &c1, double &c2, int &bad)
 // instead of 5.6, as a copy over the two layouts does (6.3 against 5.7).
 // The arrays hold a whole number of 1024-cell chunks, so blocks never split.
 constexpr int POP_BLOCK_LOG2 = 7;
-// the element index of population k of compact cell n (Q * stride < 2^32)
+// the element index of population k of compact cell n (for Q * stride < 2^32)
 template<int Q> FP_HD uint32_t pop_at(int k, uint32_t n)
 {
     return (n >> POP_BLOCK_LOG2) * (uint32_t)(Q << POP_BLOCK_LOG2) + ((uint32_t)k << POP_BLOCK_LOG2) +
            (n & ((1u << POP_BLOCK_LOG2) - 1u));
+}
+// the same in 64 bits, for arrays of 2^32 elements and more (compact cell
+// indices stay 32-bit: fewer than 2^32 fluid cells)
+template<int Q> FP_HD uint64_t pop_at64(int k, uint64_t n)
+{
+    return (n >> POP_BLOCK_LOG2) * (uint64_t)(Q << POP_BLOCK_LOG2) + ((uint64_t)k << POP_BLOCK_LOG2) +
+           (n & ((1u << POP_BLOCK_LOG2) - 1u));
+}
+// pop_at in the offset type O (uint32_t or uint64_t)
+template<int Q, class O> FP_HD O pop_at_as(int k, uint32_t n)
+{
+    if constexpr(sizeof(O) == 8) return pop_at64<Q>(k, n);
+    else return pop_at<Q>(k, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,12 +243,12 @@ template<class L> FP_HD int rowtab_words() { return 2 * RowSlots<L>::N; }
 // to): pop_at(opp k, neighbour) when the neighbour at cell - e_k is fluid,
 // else (halfway bounce-back) pop_at(k, i).  P[m]: the rank of x in slot m's
 // row (P[0]: the cell's own); start/last: the row table entries of its row.
-template<class L>
+template<class L, class O = uint32_t>
 FP_HD void compact_offsets(uint32_t i, uint32_t flags, const uint32_t (&P)[RowSlots<L>::N],
                            const uint32_t (&start)[RowSlots<L>::N],
-                           const uint32_t (&last)[RowSlots<L>::N], uint32_t (&off)[L::Q])
+                           const uint32_t (&last)[RowSlots<L>::N], O (&off)[L::Q])
 {
-    off[0] = pop_at<L::Q>(0, i);
+    off[0] = pop_at_as<L::Q, O>(0, i);
     const bool x0 = (flags & CFLAG_X0) != 0, xl = (flags & CFLAG_XLAST) != 0;
 #pragma unroll
     for(int k = 1; k < L::Q; k++) {
@@ -246,9 +259,16 @@ FP_HD void compact_offsets(uint32_t i, uint32_t flags, const uint32_t (&P)[RowSl
         if(L::ex(k) == 0) rank = P[m];
         else if(L::ex(k) > 0) rank = x0 ? last[m] : P[m] - 1u;
         else rank = xl ? 0u : P[m] + ((flags >> (3 * m + 1)) & 1u);
-        off[k] = fluid ? pop_at<L::Q>(L::opp(k), start[m] + rank) : pop_at<L::Q>(k, i);
+        off[k] = fluid ? pop_at_as<L::Q, O>(L::opp(k), start[m] + rank) : pop_at_as<L::Q, O>(k, i);
     }
 }
+
+// How the step kernels address populations.  AD_B32: 32-bit byte offsets from
+// the instance's base (its arrays span less than 4 GiB); AD_E32: 32-bit
+// element indices on a 64-bit pointer (fewer than 2^32 elements); AD_E64:
+// 64-bit element indices (any size).  A batch uses the narrowest mode that
+// covers all its instances.
+enum Addr : int { AD_B32 = 0, AD_E32 = 1, AD_E64 = 2 };
 
 enum Prec : int {
     PREC_F64 = 0,       // f in double, the reference's arithmetic

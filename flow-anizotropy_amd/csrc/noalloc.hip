@@ -179,13 +179,13 @@ extern "C" int fp_solve_dir3d_noalloc(fp_ctx *ctx, const uint8_t *solid, int nx,
         for(int step = 0; step < chunk; step++) {
             const bool local = (step & 1) == 0;
             if(local) {
-                kern_step(3, prec, true, tu, true, d_inst, d_state, base + L.lwork, n_l,
+                kern_step(3, prec, true, tu, AD_B32, d_inst, d_state, base + L.lwork, n_l,
                           (Partial *)(base + L.lpart), s);
                 kern_finish(n_lr, (const RedWork *)(base + L.lredw), d_inst, d_cfg, d_state,
                             (const Partial *)(base + L.lpart), (RPartial *)(base + L.lred),
                             (unsigned int *)(base + L.count), true, s);
             } else {
-                kern_step(3, prec, false, tu, true, d_inst, d_state, base + L.nwork, n_n,
+                kern_step(3, prec, false, tu, AD_B32, d_inst, d_state, base + L.nwork, n_n,
                           (Partial *)(base + L.npart), s);
                 kern_finish(n_nr, (const RedWork *)(base + L.nredw), d_inst, d_cfg, d_state,
                             (const Partial *)(base + L.npart), (RPartial *)(base + L.nred),
