@@ -256,6 +256,8 @@ def serving_sample(ctx, n, seed, workers, load=0.8):
         out = serve_schedule(srv, reqs, arr, SOLVE_CFG)
         st = srv.stats()
     lat = np.array([o["latency"] for o in out])
+    wait = np.array([o["t_admit"] - o["arrival"] for o in out])      # queueing before admission
+    service = np.array([o["t_done"] - o["t_admit"] for o in out])    # admission to result
     span = max(o["t_done"] for o in out) - float(arr[0])
     rows64, t64 = plain_fp64(ctx, reqs)
     err = max(k_error(o["row"], r) for o, r in zip(out, rows64))
@@ -264,6 +266,10 @@ def serving_sample(ctx, n, seed, workers, load=0.8):
         "capacity_solves_per_s": capacity,
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
         "max_latency_s": float(lat.max()), "served_solves_per_s": n / span,
+        "p50_wait_s": float(np.percentile(wait, 50)), "p99_wait_s": float(np.percentile(wait, 99)),
+        "p99_service_s": float(np.percentile(service, 99)),
+        "latency_note": "latency = wait (arrival to admission) + service (admission to result); "
+                        "the tail is the service time of the largest low-porosity GRF requests",
         "peak_device_bytes": st["bytes_peak"],
         "plain_fp64_solves_per_s": n / t64, "capacity_vs_plain_fp64": capacity * t64 / n,
         "max_k_error_vs_fp64": err,
@@ -400,7 +406,14 @@ def large_lattice(ctx, seed, steps, solve):
         res["solve"] = {"seconds": row["wall_s"],
                         "steps": [row[f"steps_{t}"] for t in ("fx", "fy", "fz")],
                         "converged": [row[f"conv_{t}"] for t in ("fx", "fy", "fz")],
-                        "K": k_matrix(row).tolist(), "k_mean": row["k_mean"], "fa": row["fa"]}
+                        "K": k_matrix(row).tolist(), "k_mean": row["k_mean"], "fa": row["fa"],
+                        "last_check": {t: {"step": row[f"check_{t}"]["step"],
+                                           "drift_over_tol": row[f"check_{t}"]["ratio"][0],
+                                           "hold": row[f"check_{t}"]["hold"]}
+                                       for t in ("fx", "fy", "fz")},
+                        "last_check_note": "drift/tol at the last first-phase check (the "
+                                           "reference's progress line): how far from the eps "
+                                           "test a run that reached max_steps stopped"}
         log(f"512^3 solve: {res['solve']}")
     return res
 

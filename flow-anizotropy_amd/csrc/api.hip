@@ -2,6 +2,7 @@
 #include "solve_common.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -114,6 +115,15 @@ double field_residual(int lx, int ly, const float *cur, const float *snap)
     return den > 0.0 ? (double)(float)std::sqrt(num / den) : 10.0;
 }
 
+static void fill_check(const InstState &S, fp_solve_out *o)
+{
+    o->check_step = S.chk_s;
+    o->checks = S.chk_n;
+    for(int k = 0; k < 3; k++) o->check_q[k] = S.chk_q[k];
+    for(int k = 0; k < 2; k++) o->check_ratio[k] = S.chk_r[k];
+    o->check_hold = S.chk_hold;
+}
+
 void fill_out(const Job &j, fp_solve_out *o)
 {
     std::memset(o, 0, sizeof(*o));
@@ -121,18 +131,43 @@ void fill_out(const Job &j, fp_solve_out *o)
     o->steps = j.state.steps;
     o->converged = j.state.converged;
     o->status = j.state.status == 2 ? FP_ERR_NONFINITE : FP_OK;
+    fill_check(j.state, o);
 }
 
 // Run a structure's forced solves to completion: together on one batch, or one
 // after another.  The jobs keep their device buffers (the caller reads the
 // velocity fields, then frees them with free_job_memory).
-std::string run_jobs(fp_ctx *ctx, std::vector<Job *> &jobs, int lattice, int prec, bool together)
+std::string run_jobs(fp_ctx *ctx, std::vector<Job *> &jobs, int lattice, int prec, bool together,
+                     int run0)
 {
     int64_t maxsteps = 0;
     for(Job *j : jobs)
         maxsteps = std::max<int64_t>(maxsteps, std::max<int64_t>(0, j->cfg.max_steps) +
                                                    std::max<int64_t>(0, j->cfg.avg_steps));
-    const int chunk = (int)std::min<int64_t>(200, std::max<int64_t>(2, (maxsteps + 1) & ~1ll));
+    int chunk = (int)std::min<int64_t>(200, std::max<int64_t>(2, (maxsteps + 1) & ~1ll));
+    // with a progress callback a chunk is no longer than the check interval,
+    // so that every check is reported (a chunk has an even length: with
+    // lag 1 only every other check is)
+    std::vector<int64_t> reported(jobs.size(), 0);
+    if(ctx->progress) {
+        int64_t lag = INT64_MAX;
+        for(Job *j : jobs) lag = std::min<int64_t>(lag, j->cfg.lag);
+        chunk = (int)std::max<int64_t>(2, std::min<int64_t>(chunk, lag) & ~1ll);
+    }
+    auto report = [&]() {
+        if(!ctx->progress) return;
+        for(size_t r = 0; r < jobs.size(); r++) {
+            const InstState &S = jobs[r]->state;
+            if(S.chk_n <= reported[r]) continue;
+            reported[r] = S.chk_n;
+            fp_solve_out o;
+            std::memset(&o, 0, sizeof(o));
+            o.steps = S.s;
+            o.converged = S.converged;
+            fill_check(S, &o);
+            ctx->progress(ctx->progress_user, run0 + (int)r, &o);
+        }
+    };
     auto step_until_done = [&](const std::vector<Job *> &set) -> std::string {
         Batch b(lattice, prec, ctx->stream, std::min(chunk, ctx->chunk), true);
         b.set_tuning(ctx->tune);
@@ -147,6 +182,7 @@ std::string run_jobs(fp_ctx *ctx, std::vector<Job *> &jobs, int lattice, int pre
             if(all) break;
             err = b.run_chunk();
             (void)b.take_done();
+            report();
         }
         b.detach_all();
         return err;
@@ -374,6 +410,14 @@ int fp_exit_code(int status)
 
 int fp_abi_version(void) { return FP_ABI_VERSION; }
 
+int fp_set_progress(fp_ctx *ctx, fp_progress_fn fn, void *user)
+{
+    if(!ctx) return FP_ERR_ARG;
+    ctx->progress = fn;
+    ctx->progress_user = fn ? user : nullptr;
+    return FP_OK;
+}
+
 fp_ctx *fp_create(int device, int *status)
 {
     fp_ctx *c = new(std::nothrow) fp_ctx;
@@ -517,7 +561,7 @@ int fp_check_mask3d(const uint8_t *solid, int nx, int ny, int nz, int64_t *n_por
 // ---------------------------------------------------------------------------
 static int solve3d_dirs(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
                         const fp_config3d *cfg, const double (*F)[3], int ndir,
-                        fp_solve_out *outs, float *vel, int64_t *updates)
+                        fp_solve_out *outs, float *vel, int64_t *updates, int run0 = 0)
 {
     if(!ctx) return FP_ERR_ARG;
     if(!cfg3_ok(cfg)) return fail(ctx, FP_ERR_ARG, "lag must be positive");
@@ -543,7 +587,7 @@ static int solve3d_dirs(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int n
         j.want_fields = vel != nullptr;
         ptrs.push_back(&j);
     }
-    e = run_jobs(ctx, ptrs, 3, cfg->precision, cfg->concurrent != 0);
+    e = run_jobs(ctx, ptrs, 3, cfg->precision, cfg->concurrent != 0, run0);
     int rc = FP_OK;
     if(!e.empty()) rc = fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
     int64_t upd = 0;
@@ -597,7 +641,7 @@ int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
         const double c = 1.0 / std::sqrt(3.0);
         const double Fv[1][3] = {{F * c, F * c, F * c}};
         int64_t u2 = 0;
-        rc = solve3d_dirs(ctx, solid, nx, ny, nz, cfg, Fv, 1, &out->val, nullptr, &u2);
+        rc = solve3d_dirs(ctx, solid, nx, ny, nz, cfg, Fv, 1, &out->val, nullptr, &u2, 3);
         if(rc != FP_OK) return rc;
         if(out->val.status != FP_OK)
             return fail(ctx, FP_ERR_NONFINITE, "non-finite flux in run val at step " +
@@ -616,7 +660,8 @@ int fp_solve3d(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz,
 // 2D
 // ---------------------------------------------------------------------------
 static int solve2d_dirs(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const fp_config2d *cfg,
-                        const double (*F)[2], int ndir, fp_solve_out *outs, float *vel, int64_t *updates)
+                        const double (*F)[2], int ndir, fp_solve_out *outs, float *vel, int64_t *updates,
+                        int run0 = 0)
 {
     if(!ctx) return FP_ERR_ARG;
     if(!cfg2_ok(cfg)) return fail(ctx, FP_ERR_ARG, "--lag and --avg-sample must be positive");
@@ -643,7 +688,7 @@ static int solve2d_dirs(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const
         j.want_fields = true;
         ptrs.push_back(&j);
     }
-    e = run_jobs(ctx, ptrs, 2, cfg->precision, cfg->concurrent != 0);
+    e = run_jobs(ctx, ptrs, 2, cfg->precision, cfg->concurrent != 0, run0);
     int rc = FP_OK;
     if(!e.empty()) rc = fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
     int64_t upd = 0;
@@ -694,7 +739,7 @@ int fp_solve2d(fp_ctx *ctx, const uint8_t *solid, int lx, int ly, const fp_confi
         validate2d_force(F, cfg->validate_deg, Fv2);
         const double Fv[1][2] = {{Fv2[0], Fv2[1]}};
         int64_t u2 = 0;
-        rc = solve2d_dirs(ctx, solid, lx, ly, cfg, Fv, 1, &out->val, nullptr, &u2);
+        rc = solve2d_dirs(ctx, solid, lx, ly, cfg, Fv, 1, &out->val, nullptr, &u2, 2);
         if(rc != FP_OK) return rc;
         if(out->val.status != FP_OK)
             return fail(ctx, FP_ERR_NONFINITE, "non-finite flux while averaging run val");

@@ -331,6 +331,14 @@ struct InstState {
     double  bs[2], bss[2], pm[2], pse[2];
     int64_t bn;
     double  q_last[3];      // flux of the last flux step
+    // the last first-phase check, as the reference's progress line prints it
+    // (LMB3d/lbm3d-perm.cpp:353-357, LMB2d/main.cpp:167-172): its step, the
+    // flux (3D) or block means (2D), drift/tol (3D) or dx/tol, dy/tol (2D),
+    // hold after it; chk_n counts the checks
+    int64_t chk_s, chk_n;
+    double  chk_q[3], chk_r[2];
+    int     chk_hold;
+    int     chk_pad;
     // results
     double  q[3], se[3];
     int64_t steps;
@@ -431,6 +439,12 @@ FP_HD void advance_state(const InstCfg &c, InstState &S, const double *q, int ha
                 for(int k = 0; k < 3; k++) drift = dmax_ref(drift, fabs(q[k] - S.qprev[k]));
                 if(S.have_prev && s >= c.min_steps && drift < c.eps * norm) S.hold++;
                 else                                                       S.hold = 0;
+                S.chk_s = s;
+                S.chk_n += 1;
+                for(int k = 0; k < 3; k++) S.chk_q[k] = q[k];
+                S.chk_r[0] = norm > 0 ? drift / (c.eps * norm) : 0.0;
+                S.chk_r[1] = 0.0;
+                S.chk_hold = S.hold;
                 for(int k = 0; k < 3; k++) S.qprev[k] = q[k];
                 S.have_prev = 1;
                 if(S.hold >= c.hold) end = true;
@@ -462,18 +476,25 @@ FP_HD void advance_state(const InstCfg &c, InstState &S, const double *q, int ha
                 const double vy = (S.bss[1] - bn*my*my) / (bn - 1);
                 const double sex = sqrt(dmax_ref(vx, 0.0) / bn);
                 const double sey = sqrt(dmax_ref(vy, 0.0) / bn);
+                double dx = 0.0, dy = 0.0, tolx = 0.0, toly = 0.0;
                 if(S.have_prev) {
                     const double qmag = sqrt(mx*mx + my*my);
                     const double flo  = c.qfloor * qmag;
-                    const double dx = fabs(mx - S.pm[0]);
-                    const double dy = fabs(my - S.pm[1]);
-                    const double tolx = dmax_ref(c.eps * dmax_ref(fabs(mx), flo),
-                                                 c.ksigma * sqrt(sex*sex + S.pse[0]*S.pse[0]));
-                    const double toly = dmax_ref(c.eps * dmax_ref(fabs(my), flo),
-                                                 c.ksigma * sqrt(sey*sey + S.pse[1]*S.pse[1]));
+                    dx = fabs(mx - S.pm[0]);
+                    dy = fabs(my - S.pm[1]);
+                    tolx = dmax_ref(c.eps * dmax_ref(fabs(mx), flo),
+                                    c.ksigma * sqrt(sex*sex + S.pse[0]*S.pse[0]));
+                    toly = dmax_ref(c.eps * dmax_ref(fabs(my), flo),
+                                    c.ksigma * sqrt(sey*sey + S.pse[1]*S.pse[1]));
                     if(s >= c.min_steps && dx < tolx && dy < toly) S.hold++;
                     else                                           S.hold = 0;
                 }
+                S.chk_s = s;
+                S.chk_n += 1;
+                S.chk_q[0] = mx; S.chk_q[1] = my; S.chk_q[2] = 0.0;
+                S.chk_r[0] = tolx > 0 ? dx / tolx : 0.0;
+                S.chk_r[1] = toly > 0 ? dy / toly : 0.0;
+                S.chk_hold = S.hold;
                 S.pm[0] = mx; S.pm[1] = my; S.pse[0] = sex; S.pse[1] = sey; S.have_prev = 1;
                 S.bs[0] = S.bs[1] = S.bss[0] = S.bss[1] = 0.0; S.bn = 0;
                 if(S.hold >= c.hold) end = true;
@@ -528,6 +549,25 @@ struct RedWork {
     int32_t inst;
     int32_t p0, p1;           // step-kernel partials [p0, p1)
     int32_t pad;
+};
+
+// The deferred finish.  When every instance of a batch has at most
+// REDUCE_SPAN partials per step kernel, a step kernel applies the previous
+// step's finish itself instead of a finish launch after every step: each
+// workgroup reduces its instance's partials of the previous step (on flux
+// steps) and runs the convergence update, all workgroups of an instance
+// computing the same state from the same inputs; the workgroup of the
+// instance's first work item writes that state for the next step.  The
+// states alternate between two arrays (a step reads one and writes the
+// other), the partials between the local and the neighbour arrays; a chunk
+// ends with one finish launch.  cfg == nullptr: every step is followed by a
+// finish launch and the step kernel reads the current state.
+struct Defer {
+    const InstCfg *cfg;       // the instances' configs, or null (no deferral)
+    InstState *sout;          // the state after the previous step, for the next step
+    const Partial *prev;      // the previous step's partials (the other kernel kind's array)
+    int32_t pending;          // 0: the state read is already current (first step of a chunk)
+    int32_t slot0;            // partial slot of this launch's item 0
 };
 
 // launch shapes of the step kernels (chosen by the autotuner)

@@ -14,6 +14,8 @@ struct fp_ctx {
     std::string err;
     fp::Tuning tune;
     int chunk = 200;
+    fp_progress_fn progress = nullptr;   // called after every convergence check (fp_set_progress)
+    void *progress_user = nullptr;
 };
 
 namespace fp {

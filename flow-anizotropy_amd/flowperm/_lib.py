@@ -40,7 +40,13 @@ class Config3D(C.Structure):
 
 class SolveOut(C.Structure):
     _fields_ = [("q", C.c_double * 3), ("se", C.c_double * 3), ("steps", C.c_int64),
-                ("converged", C.c_int32), ("status", C.c_int32), ("fieldres", C.c_double)]
+                ("converged", C.c_int32), ("status", C.c_int32), ("fieldres", C.c_double),
+                ("check_step", C.c_int64), ("checks", C.c_int64), ("check_q", C.c_double * 3),
+                ("check_ratio", C.c_double * 2), ("check_hold", C.c_int32), ("pad", C.c_int32)]
+
+
+# void (*fp_progress_fn)(void *user, int run, const fp_solve_out *out)
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(SolveOut))
 
 
 class Result3D(C.Structure):
@@ -120,6 +126,7 @@ _ip = C.POINTER(C.c_int)
 SIGNATURES = {
     "fp_exit_code": (C.c_int, [C.c_int]),
     "fp_abi_version": (C.c_int, []),
+    "fp_set_progress": (C.c_int, [C.c_void_p, PROGRESS_FN, C.c_void_p]),
     "fp_create": (C.c_void_p, [C.c_int, _ip]),
     "fp_destroy": (None, [C.c_void_p]),
     "fp_last_error": (C.c_char_p, [C.c_void_p]),

@@ -167,6 +167,37 @@ SPEC2 = {"--ff": ("ff", "d"), "--eps": ("eps", "d"), "--lag": ("lag", "l"),
          "--precision": ("precision", "s")}
 
 
+def _progress_lines(fmt):
+    """a progress callback collecting each run's check lines, and a function
+    printing them run by run (the order the reference's sequential runs print
+    them in)"""
+    lines = {}
+
+    def progress(run, c):
+        lines.setdefault(run, []).append(fmt(run, c))
+
+    def flush():
+        for run in sorted(lines):
+            for line in lines[run]:
+                print(line)
+        sys.stdout.flush()
+    return progress, flush
+
+
+def _line3d(run, c):
+    """LMB3d/lbm3d-perm.cpp:353-357"""
+    q = c["q"]
+    return (f"# {('fx', 'fy', 'fz', 'val')[run]} {c['step']}  q=({q[0]:g}, {q[1]:g}, {q[2]:g})  "
+            f"drift/tol={c['ratio'][0]:g}  hold={c['hold']}")
+
+
+def _line2d(run, c):
+    """LMB2d/main.cpp:167-172"""
+    q, r = c["q"], c["ratio"]
+    return (f"# {('fx', 'fy', 'val')[run]} {c['step']}  qx={q[0]:g}  qy={q[1]:g}  "
+            f"dx/tol={r[0]:g}  dy/tol={r[1]:g}  hold={c['hold']}")
+
+
 def lbm3d_main(argv):
     from . import structio
     from ._lib import FlowpermError
@@ -184,10 +215,14 @@ def lbm3d_main(argv):
         F = 2.5e-7 * cfg.get("ff", 1.0)
         print(f"# structure {structure}  {nx}x{ny}x{nz}  porosity {1 - mask.mean():g}  "
               f"force {F:g}  mu {1.0 / 6.0:g}", flush=True)
+    progress, flush = _progress_lines(_line3d)
     try:
-        row = solve_permeability3d(mask, filename=structure, velocity=prefix is not None, **cfg)
+        row = solve_permeability3d(mask, filename=structure, velocity=prefix is not None,
+                                   progress=None if quiet else progress, **cfg)
     except FlowpermError as e:
+        flush()
         raise _Die(str(e).rsplit(" (status", 1)[0])
+    flush()
     if prefix is not None:
         for d, t in enumerate(("fx", "fy", "fz")):
             _write_vel3d(f"{prefix}.{t}.vel", row["velocity"][d])
@@ -230,10 +265,19 @@ def lbm2d_main(argv):
         mask = structio.load_structure2d(structure)
     except (OSError, ValueError) as e:
         raise _Die(str(e))
+    if not quiet:                                 # main.cpp:317-319 (porosity and mu are floats)
+        por = np.float32((mask == 0).sum()) / np.float32(mask.size)
+        ff = cfg.get("ff", 1.0)
+        print(f"# structure {structure}  porosity {float(por):g}  force {2.5e-7 * ff:g} "
+              f"(ff={ff:g})  mu {float(np.float32(0.33 * 0.5)):g}", flush=True)
+    progress, flush = _progress_lines(_line2d)
     try:
-        row = solve_permeability2d(mask, filename=structure, velocity=prefix is not None, **cfg)
+        row = solve_permeability2d(mask, filename=structure, velocity=prefix is not None,
+                                   progress=None if quiet else progress, **cfg)
     except FlowpermError as e:
+        flush()
         raise _Die(str(e).rsplit(" (status", 1)[0])
+    flush()
     if prefix is not None:
         for d, t in enumerate(("fx", "fy")):
             _write_vel2d(f"{prefix}.{t}.dat", row["velocity"][d])

@@ -148,6 +148,33 @@ def _raise(ctx, st):
     raise FlowpermError(st, ctx.last_error())
 
 
+def _check_of(o):
+    """the last first-phase convergence check of a run (fp_solve_out check_*)"""
+    return {"step": int(o.check_step), "checks": int(o.checks), "q": list(o.check_q),
+            "ratio": list(o.check_ratio), "hold": int(o.check_hold)}
+
+
+def _call(ctx, progress, fn, *args):
+    """fn(*args) under the context's lock; with progress (a callable taking
+    the run index and a check dict, plus 'steps' and 'converged') called after
+    every first-phase convergence check (fp_set_progress)"""
+    with ctx.lock:
+        if progress is None:
+            return fn(*args)
+
+        def cb(user, run, out):
+            o = out.contents
+            d = _check_of(o)
+            d["steps"], d["converged"] = int(o.steps), int(o.converged)
+            progress(int(run), d)
+        hook = _lib.PROGRESS_FN(cb)
+        ctx.lib.fp_set_progress(ctx.handle, hook, None)
+        try:
+            return fn(*args)
+        finally:
+            ctx.lib.fp_set_progress(ctx.handle, _lib.PROGRESS_FN(), None)
+
+
 def jacobi3(A):
     """The reference's cyclic Jacobi (LMB3d/lbm3d-perm.cpp:401-443) via the library."""
     lib = _lib.load()
@@ -158,8 +185,9 @@ def jacobi3(A):
     return val, vec.reshape(3, 3)
 
 
-def solve_direction3d(mask_or_path, F, *, ctx=None, velocity=False, **cfg):
-    """One forced solve (solve(), LMB3d/lbm3d-perm.cpp:320-398)."""
+def solve_direction3d(mask_or_path, F, *, ctx=None, velocity=False, progress=None, **cfg):
+    """One forced solve (solve(), LMB3d/lbm3d-perm.cpp:320-398).  progress:
+    see solve_permeability3d."""
     ctx = ctx or default_context()
     mask, _ = _mask3d(mask_or_path)
     nz, ny, nx = mask.shape
@@ -167,28 +195,32 @@ def solve_direction3d(mask_or_path, F, *, ctx=None, velocity=False, **cfg):
     out = _lib.SolveOut()
     Fa = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(3))
     vel = np.zeros((nz, ny, nx, 3), np.float32) if velocity else None
-    with ctx.lock:
-        st = ctx.lib.fp_solve_dir3d(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, C.byref(c),
-                                    _lib.f64ptr(Fa), C.byref(out), _lib.f32ptr(vel))
+    st = _call(ctx, progress, ctx.lib.fp_solve_dir3d, ctx.handle, _lib.u8ptr(mask), nx, ny, nz,
+               C.byref(c), _lib.f64ptr(Fa), C.byref(out), _lib.f32ptr(vel))
     if st not in (_lib.FP_OK, _lib.FP_NOT_CONVERGED):
         _raise(ctx, st)
     res = {"q": np.array(out.q[:]), "se": np.array(out.se[:]), "steps": int(out.steps),
-           "converged": int(out.converged), "status": st}
+           "converged": int(out.converged), "status": st, "check": _check_of(out)}
     if velocity:
         res["velocity"] = vel
     return res
 
 
-def solve_permeability3d(mask_or_path, *, ctx=None, filename=None, velocity=False, **cfg):
+def solve_permeability3d(mask_or_path, *, ctx=None, filename=None, velocity=False, progress=None,
+                         **cfg):
     """One lbm3d-perm invocation on a mask (array [z,y,x] or .raw path).
 
     Keyword arguments: ff, eps, lag, hold, min_steps, max_steps, avg_steps
     (the reference flags), validate, precision ("f64" | "f32"), concurrent (default
     True: the three forcing runs step together; the results are bitwise those of
     running them one after another, as the reference does).
-    Returns the CSV row as a dict plus 'exit_code', 'status' and, with
-    validate, 'val_*' entries; with velocity=True also 'velocity'
-    (3, nz, ny, nx, 3) float32.
+    Returns the CSV row as a dict plus 'exit_code', 'status', the last
+    first-phase check of every run ('check_fx' ...: step, q, ratio =
+    [drift/tol, 0], hold) and, with validate, 'val_*' entries; with
+    velocity=True also 'velocity' (3, nz, ny, nx, 3) float32.  progress, when
+    given, is called as progress(run, check) after every first-phase check of
+    a run (run 0..2 = fx, fy, fz, 3 = val; check as above plus 'steps' and
+    'converged'): the values of the reference's per-check progress line.
     """
     ctx = ctx or default_context()
     mask, name = _mask3d(mask_or_path)
@@ -196,9 +228,8 @@ def solve_permeability3d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
     c = config3d(**cfg)
     r = _lib.Result3D()
     vel = np.zeros((3, nz, ny, nx, 3), np.float32) if velocity else None
-    with ctx.lock:
-        st = ctx.lib.fp_solve3d(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, C.byref(c),
-                                C.byref(r), _lib.f32ptr(vel))
+    st = _call(ctx, progress, ctx.lib.fp_solve3d, ctx.handle, _lib.u8ptr(mask), nx, ny, nz,
+               C.byref(c), C.byref(r), _lib.f32ptr(vel))
     if st not in (_lib.FP_OK, _lib.FP_NOT_CONVERGED):
         _raise(ctx, st)
     row = row3d(r, st, c.ff, filename if filename is not None else name)
@@ -217,6 +248,8 @@ def row3d(r, st, ff, filename):
     for d, t in enumerate(_TAGS):
         row[f"steps_{t}"] = int(r.run[d].steps)
         row[f"conv_{t}"] = int(r.run[d].converged)
+    for d, t in enumerate(_TAGS):
+        row[f"check_{t}"] = _check_of(r.run[d])
     for d, t in enumerate(_TAGS):
         for k, a in enumerate(_AX):
             row[f"q{a}_{t}"] = r.run[d].q[k]
@@ -276,12 +309,16 @@ def _mask2d(mask_or_path):
     return np.ascontiguousarray(arr.astype(np.uint8, copy=False)), "<array>"
 
 
-def solve_permeability2d(mask_or_path, *, ctx=None, filename=None, velocity=False, **cfg):
+def solve_permeability2d(mask_or_path, *, ctx=None, filename=None, velocity=False, progress=None,
+                         **cfg):
     """One lbm2d-perm invocation on a mask ([y, x] array or GIF/PPM/DAT path).
 
     Keyword arguments: ff, eps, lag, hold, min_steps, max_steps, avg_steps,
     avg_sample, qfloor, ksigma, validate (degrees), L0, precision ("f32ref" =
     the reference's float arithmetic, "f64", "f32" = shifted), concurrent.
+    The row carries the last first-phase check of each run ('check_fx',
+    'check_fy': step, q = block means, ratio = [dx/tol, dy/tol], hold);
+    progress(run, check) is called after every check (run 0 fx, 1 fy, 2 val).
     """
     ctx = ctx or default_context()
     mask, name = _mask2d(mask_or_path)
@@ -289,9 +326,8 @@ def solve_permeability2d(mask_or_path, *, ctx=None, filename=None, velocity=Fals
     c = config2d(shape=mask.shape, **cfg)
     r = _lib.Result2D()
     vel = np.zeros((2, ly, lx, 2), np.float32) if velocity else None
-    with ctx.lock:
-        st = ctx.lib.fp_solve2d(ctx.handle, _lib.u8ptr(mask), lx, ly, C.byref(c), C.byref(r),
-                                _lib.f32ptr(vel))
+    st = _call(ctx, progress, ctx.lib.fp_solve2d, ctx.handle, _lib.u8ptr(mask), lx, ly, C.byref(c),
+               C.byref(r), _lib.f32ptr(vel))
     if st not in (_lib.FP_OK, _lib.FP_NOT_CONVERGED):
         _raise(ctx, st)
     row = row2d(r, st, c.ff, filename if filename is not None else name)
@@ -315,6 +351,7 @@ def row2d(r, st, ff, filename):
            "k1": r.k1, "k2": r.k2, "k_mean": r.k_mean, "anisotropy": r.anisotropy,
            "theta_deg": r.theta_deg, "alpha_x_deg": r.alpha_x_deg, "alpha_y_deg": r.alpha_y_deg,
            "fieldres_fx": rx.fieldres, "fieldres_fy": ry.fieldres,
+           "check_fx": _check_of(rx), "check_fy": _check_of(ry),
            "exit_code": _lib.exit_code(st), "status": st, "seconds": r.seconds,
            "fluid_updates": int(r.fluid_updates)}
     if r.validated:

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 6
+#define FP_ABI_VERSION 7
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -100,7 +100,31 @@ typedef struct fp_solve_out {
     int32_t converged;
     int32_t status;         /* FP_OK or FP_ERR_NONFINITE                      */
     double  fieldres;       /* 2D only: field residual (LMB2d/lbm.cpp:169-182) */
+    /* The last first-phase convergence check, the values the reference's
+     * progress line prints (LMB3d/lbm3d-perm.cpp:353-357 "# fx S q=(..)
+     * drift/tol=R hold=H"; LMB2d/main.cpp:167-172 "# fx S qx=.. qy=..
+     * dx/tol=.. dy/tol=.. hold=H").  For a run that reached max_steps,
+     * check_ratio says how far from converged it stopped. */
+    int64_t check_step;     /* step of that check; 0 = no check ran           */
+    int64_t checks;         /* checks in the first phase so far               */
+    double  check_q[3];     /* 3D: the flux q; 2D: the block means qx, qy     */
+    double  check_ratio[2]; /* 3D: drift/tol in [0]; 2D: dx/tol, dy/tol       */
+    int32_t check_hold;     /* hold after that check                          */
+    int32_t pad;
 } fp_solve_out;
+
+/* Progress of fp_solve3d / fp_solve2d / fp_solve_dir3d / fp_solve_dir2d on
+ * this context: after every first-phase convergence check of a run, fn(user,
+ * run, out) with the run's index in the call (0 fx, 1 fy, 2 fz or 2D val,
+ * 3 3D val) and its check_* fields, steps (the run's step count) and
+ * converged filled.  It replaces the per-check progress lines of the
+ * reference (LMB3d/lbm3d-perm.cpp:353-357, LMB2d/main.cpp:167-172), which the
+ * executables print from it unless --quiet.  While a callback is set, the
+ * runs are stepped in chunks no longer than the check interval (lag), so
+ * every check is reported (with lag 1, every other one).  NULL turns it off.
+ * The callback runs on the calling thread, between chunks. */
+typedef void (*fp_progress_fn)(void *user, int run, const fp_solve_out *out);
+int     fp_set_progress(fp_ctx *ctx, fp_progress_fn fn, void *user);
 
 /* Everything lbm3d-perm writes to its CSV row (LMB3d/lbm3d-perm.cpp:574-615)
  * plus the --validate tensor test (:555-572). */

@@ -5,6 +5,7 @@ handling, exit codes and CSV text of the reference executables
 on the CPU (they fail before any device work); full runs are GPU tests.
 """
 import csv
+import re
 import subprocess
 import sys
 
@@ -85,6 +86,76 @@ def test_lbm3d_row_matches_the_reference_row(tmp_path):
     assert check_eigen_and_reciprocity(ours.__getitem__, refd, trace, name) >= 1
     rc2, _, _ = run("lbm3d-perm", GOLDEN / f"{name}.raw", out, *argv)
     assert rc2 == rc and len(out.read_text().splitlines()) == 3
+
+
+def test_progress_line_text():
+    """the per-check progress lines in the reference's text: 3D from a line of
+    the reference binary's own stdout (fixture ell_x130y8z6_fast_s7), 2D as
+    LMB2d/main.cpp:167-172 prints with ostream's default 6 digits"""
+    from flowperm.cli import _line2d, _line3d
+    from conftest import golden
+    _, _, meta = golden("ell_x130y8z6_fast_s7")
+    assert meta["stdout"][0] == _line3d(0, {"step": 66000, "q": [1.38345e-06, 2.97032e-07, 1.54424e-07],
+                                            "ratio": [1.14254, 0.0], "hold": 0})
+    assert _line3d(3, {"step": 400, "q": [0.0, -1e-10, 2.5], "ratio": [0.0, 0.0], "hold": 2}) == \
+        "# val 400  q=(0, -1e-10, 2.5)  drift/tol=0  hold=2"
+    assert _line2d(1, {"step": 15000, "q": [-2.5e-07, 0.000123456789, 0.0], "ratio": [0.0, 0.5],
+                       "hold": 1}) == "# fy 15000  qx=-2.5e-07  qy=0.000123457  dx/tol=0  dy/tol=0.5  hold=1"
+
+
+NUM = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?")
+
+
+def assert_stdout_matches(ours, ref, name, eps):
+    """Our executable's stdout against the reference binary's, line by line:
+    the same text between the numbers, integers (steps, hold, counts) equal,
+    printed values (6 significant digits, ostream's default) within one unit of
+    the 6th digit.  Two kinds of value are differences of nearly equal numbers,
+    so the reference's reassociated (-ffast-math) sums show in them: drift/tol
+    = max |q - q_prev| / (eps |q|) is off by at most 2 dq / (eps |q|), which
+    for fluxes equal to 1e-10 relative (the bar the rows are held to) is
+    2e-10 / eps; the reciprocity residuals |K_ij - K_ji| / trace are within
+    1e-9, and the tensor test's relative error |q_val - K F| / |q_val| within
+    2e-10."""
+    a = ours.splitlines()
+    assert len(a) == len(ref), (name, len(a), len(ref))
+    for x, y in zip(a, ref):
+        text = NUM.split(y)
+        assert NUM.split(x) == text, (name, x, y)
+        for before, p, q in zip(text, NUM.findall(x), NUM.findall(y)):
+            if re.fullmatch(r"[-+]?\d+", q):
+                assert p == q, (name, x, y)
+            elif y.startswith("# reciprocity"):
+                assert abs(float(p) - float(q)) <= 1e-9, (name, x, y)
+            elif y.startswith("#   relative error"):      # |q_val - K F| / |q_val|, also a difference
+                assert abs(float(p) - float(q)) <= 1.01e-5 * abs(float(q)) + 2e-10, (name, x, y)
+            elif before.endswith("tol="):
+                assert abs(float(p) - float(q)) <= 1.01e-5 * abs(float(q)) + 2e-10 / eps, (name, x, y)
+            else:
+                assert abs(float(p) - float(q)) <= 1.01e-5 * abs(float(q)) + 1e-300, (name, x, y)
+
+
+PROGRESS_FIXTURES = ["sph16_s0", "sph20_fast_s2", "ell_x24y20z16_fast_s5", "grf_x36y20z24_fast_s8",
+                     "ell24_validate_s3"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", PROGRESS_FIXTURES)
+def test_lbm3d_stdout_matches_the_reference_binary(tmp_path, name):
+    """Without --quiet the executable prints what the reference prints: the
+    structure line, one line per convergence check of every run
+    (LMB3d/lbm3d-perm.cpp:353-357: step, q, drift/tol, hold), the tensor test
+    with --validate, K, the reciprocity residuals and the eigenvalues.  The
+    reference binary's stdout of the same invocation is in the fixture; every
+    check's flux and drift/tol must agree to the printed digits."""
+    mask, row, meta = golden(name)
+    argv = meta["argv"][3:]
+    assert "--quiet" not in argv
+    (tmp_path / f"{name}.raw").write_bytes((GOLDEN / f"{name}.raw").read_bytes())
+    rc, stdout, err = run("lbm3d-perm", f"{name}.raw", "o.csv", *argv, cwd=tmp_path)
+    assert rc == meta["returncode"], err
+    eps = float(argv[argv.index("--eps") + 1]) if "--eps" in argv else 1e-5
+    assert_stdout_matches(stdout, meta["stdout"], name, eps)
 
 
 @pytest.mark.gpu

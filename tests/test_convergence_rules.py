@@ -50,7 +50,7 @@ def replay2d_product(seq, **cfg):
     out = _lib.SolveOut()
     rc = _lib.load().fp_replay_convergence2d(C.byref(c), _lib.f64ptr(q), q.shape[0], C.byref(out))
     return rc, {"q": np.array(out.q[:2]), "se": np.array(out.se[:2]), "steps": out.steps,
-                "converged": out.converged, "status": out.status}
+                "converged": out.converged, "status": out.status, "check": check_of(out)}
 
 
 def replay3d_product(seq, **cfg):
@@ -61,7 +61,12 @@ def replay3d_product(seq, **cfg):
     out = _lib.SolveOut()
     rc = _lib.load().fp_replay_convergence3d(C.byref(c), _lib.f64ptr(q), q.shape[0], C.byref(out))
     return rc, {"q": np.array(out.q[:]), "se": np.array(out.se[:]), "steps": out.steps,
-                "converged": out.converged, "status": out.status}
+                "converged": out.converged, "status": out.status, "check": check_of(out)}
+
+
+def check_of(out):
+    return {"step": out.check_step, "checks": out.checks, "q": np.array(out.check_q[:]),
+            "ratio": np.array(out.check_ratio[:]), "hold": out.check_hold}
 
 
 def all_replays2d(oracle, seq, cfg):
@@ -210,6 +215,44 @@ def test_3d_rules_mean_of_the_averaging_phase(oracle):
     for r in (p, o):
         assert r["steps"] == 400 and r["q"][0] == A
         assert abs(r["se"][0] - se) <= 1e-15 * se
+
+
+def test_last_check_values_3d():
+    """The values of the reference's progress line at the last check
+    (LMB3d/lbm3d-perm.cpp:353-357: drift/tol = max_c |q_c - qprev_c| /
+    (eps |q|)), derived by hand.  A flux that grows by 2e-6 A per check never
+    converges: the last check is at max_steps 2000, the 20th, with q = A (1 +
+    4e-5), drift 2e-6 A against eps |q| = 1e-6 A (1 + 4e-5), so drift/tol =
+    2 / 1.00004, hold 0.  A constant flux stops at its third check (300) with
+    drift 0 and hold 2."""
+    rc, p = replay3d_product(seq3(3000, lambda s: [A * (1 + 2e-6 * (s // 100)), 0.0, 0.0]), **CFG3)
+    c = p["check"]
+    assert (c["step"], c["checks"], c["hold"]) == (2000, 20, 0)
+    assert c["q"][0] == A * (1 + 2e-6 * 20) and c["q"][1] == c["q"][2] == 0.0
+    assert abs(c["ratio"][0] - 2 / 1.00004) <= 1e-9 and c["ratio"][1] == 0.0
+    rc, p = replay3d_product(seq3(3000, lambda s: [A, -A / 2, 0.0]), **CFG3)
+    c = p["check"]
+    assert (c["step"], c["checks"], c["hold"]) == (300, 3, 2)
+    assert list(c["q"]) == [A, -A / 2, 0.0] and list(c["ratio"]) == [0.0, 0.0]
+
+
+def test_last_check_values_2d():
+    """LMB2d/main.cpp:167-172 at the last check, by hand.  Block means that
+    grow by 1e-2 A per 100-step block (samples alternating +-0.1 A about them)
+    with ksigma 0 never converge: the last check is at max_steps 2000, the
+    20th, block mean qx = 1.19 A against 1.18 A before, tol = eps |qx| =
+    1.19e-3 A (qfloor 1e-2 |m| is smaller), dx/tol = 0.01 / 1.19e-3; qy = 0
+    gives dy = 0 and dy/tol 0.  Before any previous block (the first check)
+    both ratios are 0."""
+    seq = seq2(3000, lambda s: A * (1 + 1e-2 * ((s - 1) // 100)) + A * alternating(s, 0.1))
+    rc, p = replay2d_product(seq, **dict(CFG2, ksigma=0.0))
+    c = p["check"]
+    assert (c["step"], c["checks"], c["hold"]) == (2000, 20, 0)
+    assert abs(c["q"][0] - 1.19 * A) <= 1e-14 * A and c["q"][1] == 0.0
+    assert abs(c["ratio"][0] - 0.01 / 1.19e-3) <= 1e-9 * 8.4 and c["ratio"][1] == 0.0
+    rc, p = replay2d_product(seq2(250, lambda s: A), **dict(CFG2, max_steps=150, avg_steps=0))
+    c = p["check"]
+    assert (c["step"], c["checks"], c["hold"]) == (100, 1, 0) and list(c["ratio"]) == [0.0, 0.0]
 
 
 def test_replay_rejects_a_short_sequence():
