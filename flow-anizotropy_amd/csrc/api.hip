@@ -1011,6 +1011,7 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
     e = b.time_steps(steps, kt);
     float graph_ms = 0.0f;
     if(e.empty()) e = b.graph_steps(steps, graph_ms);
+    const bool deferred = b.deferred();
     b.detach_all();
     free_job_memory(j);
     if(!e.empty()) return fail(ctx, FP_ERR_DEVICE, e);
@@ -1023,6 +1024,8 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
     out->nbr_ms = kt.n_nbr ? kt.nbr_ms / kt.n_nbr : 0;
     out->monitor_ms = kt.n_monitor ? kt.monitor_ms / kt.n_monitor : 0;
     out->bytes_per_update = (double)(lattice == 3 ? 19 : 9) * 2 * (precision == FP_F64 ? 8 : 4);
+    out->deferred = deferred ? 1 : 0;
+    out->pad = 0;
     return FP_OK;
 }
 

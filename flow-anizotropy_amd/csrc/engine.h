@@ -14,8 +14,11 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s);
 // ad: how the step kernels address populations (Addr: AD_B32 32-bit byte
 // offsets, every instance spans less than 4 GiB; AD_E32 32-bit element
 // indices; AD_E64 64-bit element indices, instances of 2^32 elements or more)
+// defer: the deferred finish of the previous step (struct Defer), or null
+// (the state st is current and a finish launch follows the step)
 void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, const InstDesc *insts,
-               const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s);
+               const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s,
+               const Defer *defer = nullptr);
 // plain copies of n16 float4: kind 0 persistent grid, 1 one workgroup per
 // 8 KiB, 2 the same with non-temporal loads and stores
 constexpr int N_COPY_KINDS = 3;
@@ -134,6 +137,8 @@ public:
     const Tuning &tuning() const { return tune_; }
     // neighbour-step forms in use: 0 row segments, 1 compact, 2 both
     int nbr_mode() const { return mode_; }
+    // whether chunks run with the deferred finish (struct Defer)
+    bool deferred() const { return defer_; }
 
     // Timing: launch `steps` steps (even) directly, with a HIP event pair on
     // the batch stream around every kernel; returns the summed kernel times
@@ -150,8 +155,11 @@ public:
 
 private:
     std::string rebuild();
-    void launch_step(bool local);
+    void launch_step(bool local, const Defer *df = nullptr);
     void launch_finish(bool local);
+    // the kernels of n steps (even) from an even step count, each followed
+    // by its finish, or with the deferred finish and one finish at the end
+    void enqueue_steps(int n);
     bool wants_rebuild() const;
     int64_t work_items(const Job &j) const;
     void release_graph();
@@ -167,6 +175,8 @@ private:
     InstDesc *d_inst_ = nullptr;
     InstCfg *d_cfg_ = nullptr;
     InstState *d_state_ = nullptr;
+    InstState *d_state2_ = nullptr;          // the other state array of the deferred finish
+    bool defer_ = false;
     LocalWork *d_lwork_ = nullptr;
     NbrWork *d_nwork_ = nullptr;
     LocalWork *d_cwork_ = nullptr;           // compact neighbour-step items (mode 1)

@@ -452,6 +452,7 @@ Batch::~Batch()
     (void)hipFree(d_inst_);
     (void)hipFree(d_cfg_);
     (void)hipFree(d_state_);
+    (void)hipFree(d_state2_);
     (void)hipFree(d_lwork_);
     (void)hipFree(d_nwork_);
     (void)hipFree(d_cwork_);
@@ -808,6 +809,13 @@ std::string Batch::rebuild()
     for(int i = 0; i < n_inst_; i++) { cfg[i] = jobs_[i]->cfg; state[i] = jobs_[i]->state; }
     n_lred_ = (int)lr.size();
     n_nred_ = (int)nr.size();
+    // the deferred finish needs every instance's partials of a step kernel
+    // in one reduction item (so that one workgroup can sum them);
+    // FLOWPERM_DEFER_FINISH=0 turns it off
+    const char *dv = getenv("FLOWPERM_DEFER_FINISH");
+    defer_ = !(dv && dv[0] == '0') && ad_ == AD_B32;
+    for(int i = 0; i < n_inst_ && defer_; i++)
+        if(desc[i].rl1 - desc[i].rl0 != 1 || desc[i].rn1 - desc[i].rn0 != 1) defer_ = false;
     if(lw.size() > INT32_MAX || nw.size() > INT32_MAX || cw.size() > INT32_MAX) return "batch too large";
     n_lwork_ = (int)lw.size();
     n_nwork_ = (int)nw.size();
@@ -815,14 +823,15 @@ std::string Batch::rebuild()
     work_total_ = (int64_t)lw.size() + (int64_t)n_nwork_ + (int64_t)n_cwork_;
     const size_t ni = (size_t)n_inst_;
     if(ni > cap_inst_ || !d_inst_) {
-        (void)hipFree(d_inst_); (void)hipFree(d_cfg_); (void)hipFree(d_state_);
-        d_inst_ = nullptr; d_cfg_ = nullptr; d_state_ = nullptr;
+        (void)hipFree(d_inst_); (void)hipFree(d_cfg_); (void)hipFree(d_state_); (void)hipFree(d_state2_);
+        d_inst_ = nullptr; d_cfg_ = nullptr; d_state_ = nullptr; d_state2_ = nullptr;
         if(h_state_) { (void)hipHostFree(h_state_); h_state_ = nullptr; }
         cap_inst_ = 0;
         const size_t c = std::max(ni, (size_t)16);
         if((e = hipMalloc(&d_inst_, c * sizeof(InstDesc))) != hipSuccess ||
            (e = hipMalloc(&d_cfg_, c * sizeof(InstCfg))) != hipSuccess ||
            (e = hipMalloc(&d_state_, c * sizeof(InstState))) != hipSuccess ||
+           (e = hipMalloc(&d_state2_, c * sizeof(InstState))) != hipSuccess ||
            (e = hipHostMalloc((void **)&h_state_, c * sizeof(InstState), hipHostMallocDefault)) != hipSuccess)
             return hip_err("allocating batch tables", e);
         cap_inst_ = c;
@@ -873,13 +882,7 @@ std::string Batch::run_chunk()
     if(n_inst_ == 0) return "";
     const double t0 = now_seconds();
     hipError_t e;
-    auto launch_all = [&]() {
-        for(int t = 0; t < chunk_; t++) {
-            const bool local = (t & 1) == 0;     // every live job starts a chunk at an even step count
-            launch_step(local);
-            launch_finish(local);
-        }
-    };
+    auto launch_all = [&]() { enqueue_steps(chunk_); };
     if(use_graph_) {
         if(!exec_) {
             e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
@@ -930,23 +933,63 @@ void Batch::launch_finish(bool local)
                 local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, d_count_, local, stream_);
 }
 
-void Batch::launch_step(bool local)
+void Batch::launch_step(bool local, const Defer *df)
 {
+    // the state the kernels read: the current one, or (deferred) the one
+    // before the previous step's finish
+    const InstState *st = d_state_;
+    Defer d = {};
+    if(df) {
+        d = *df;
+        st = d.sout == d_state2_ ? d_state_ : d_state2_;
+        d.slot0 = 0;
+    }
     if(local) {
-        kern_step(lattice_, prec_, true, tune_, ad_, d_inst_, d_state_, d_lwork_, n_lwork_, d_lpart_, stream_);
+        kern_step(lattice_, prec_, true, tune_, ad_, d_inst_, st, d_lwork_, n_lwork_, d_lpart_, stream_,
+                  df ? &d : nullptr);
         return;
     }
     // the row-segment items, then the compact ones (their partial slots follow)
     Tuning tu = tune_;
     if(n_nwork_ > 0) {
         tu.nbr_mode = 0;
-        kern_step(lattice_, prec_, false, tu, ad_, d_inst_, d_state_, d_nwork_, n_nwork_, d_npart_, stream_);
+        kern_step(lattice_, prec_, false, tu, ad_, d_inst_, st, d_nwork_, n_nwork_, d_npart_, stream_,
+                  df ? &d : nullptr);
     }
     if(n_cwork_ > 0) {
         tu.nbr_mode = 1;
-        kern_step(lattice_, prec_, false, tu, ad_, d_inst_, d_state_, d_cwork_, n_cwork_,
-                  d_npart_ + n_nwork_, stream_);
+        d.slot0 = n_nwork_;
+        kern_step(lattice_, prec_, false, tu, ad_, d_inst_, st, d_cwork_, n_cwork_,
+                  d_npart_ + n_nwork_, stream_, df ? &d : nullptr);
     }
+}
+
+void Batch::enqueue_steps(int n)
+{
+    if(!defer_) {
+        for(int t = 0; t < n; t++) {
+            const bool local = (t & 1) == 0;     // every live job starts at an even step count
+            launch_step(local);
+            launch_finish(local);
+        }
+        return;
+    }
+    // step t reads one state array and writes the other: t = 0 reads the
+    // current state (nothing pending) from d_state_ and hands it on to
+    // d_state2_; every later step applies the previous step's finish first.
+    // n is even, so the last step writes d_state_, and the finish launch of
+    // the last (neighbour) step completes it there.
+    for(int t = 0; t < n; t++) {
+        const bool local = (t & 1) == 0;
+        Defer df;
+        df.cfg = d_cfg_;
+        df.sout = (t & 1) == 0 ? d_state2_ : d_state_;
+        df.prev = local ? d_npart_ : d_lpart_;
+        df.pending = t > 0;
+        df.slot0 = 0;
+        launch_step(local, &df);
+    }
+    launch_finish(false);
 }
 
 std::string Batch::time_steps(int steps, KernelTimes &kt)
@@ -1014,11 +1057,7 @@ std::string Batch::graph_steps(int steps, float &ms)
     auto capture = [&](int n, hipGraph_t &gg, hipGraphExec_t &xx) -> std::string {
         hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
         if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
-        for(int t = 0; t < n; t++) {
-            const bool local = (t & 1) == 0;
-            launch_step(local);
-            launch_finish(local);
-        }
+        enqueue_steps(n);
         e = hipStreamEndCapture(stream_, &gg);
         if(e != hipSuccess) { gg = nullptr; return hip_err("hipStreamEndCapture", e); }
         e = hipGraphInstantiate(&xx, gg, nullptr, nullptr, 0);

@@ -154,6 +154,125 @@ __device__ __forceinline__ void block_flux(double (&c)[3][3], int bad, Partial *
     }
 }
 
+// ---------------------------------------------------------------------------
+// deferred finish (struct Defer): the state a step runs with
+// ---------------------------------------------------------------------------
+// sum of 128-bit values over a workgroup of up to 1024 threads (butterflies
+// inside each wavefront, then the wavefronts' sums in LDS); exact, result in
+// thread 0
+__device__ __forceinline__ void wg_sum_i128(I128 (&s)[3], int &bad)
+{
+    for(int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for(int k = 0; k < 3; k++) {
+            I128 o;
+            o.lo = __shfl_down(s[k].lo, off);
+            o.hi = __shfl_down(s[k].hi, off);
+            s[k] = i128_add(s[k], o);
+        }
+        bad |= __shfl_down(bad, off);
+    }
+    __shared__ I128 ws[16][3];
+    __shared__ int wb[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if(lane == 0) {
+#pragma unroll
+        for(int k = 0; k < 3; k++) ws[wv][k] = s[k];
+        wb[wv] = bad;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for(int w = 1; w < nw; w++) {
+#pragma unroll
+            for(int k = 0; k < 3; k++) s[k] = i128_add(s[k], ws[w][k]);
+            bad |= wb[w];
+        }
+    }
+}
+
+// copy of one InstState by the whole workgroup, a 32-bit word per thread
+__device__ __forceinline__ void wg_copy_state(InstState *dst, const InstState *src)
+{
+    constexpr int NW = (int)(sizeof(InstState) / 4);
+    static_assert(sizeof(InstState) % 4 == 0, "InstState is copied in 32-bit words");
+    for(int w = (int)threadIdx.x; w < NW; w += (int)blockDim.x)
+        reinterpret_cast<uint32_t *>(dst)[w] = reinterpret_cast<const uint32_t *>(src)[w];
+}
+
+// The state this step runs with for instance `inst`, into shS (LDS): st[inst]
+// itself when it is current (df.pending == 0), else with the previous step's
+// finish applied -- the reduction and update k_finish would have done, in the
+// same integer arithmetic (the sum is exact, so the order of the partials
+// does not matter) and the same floating-point operations.  PREV_LOCAL: the
+// previous step was a local step (partials [lp0, lp1)), else a neighbour step
+// ([np0, np1)).  Every thread of the workgroup calls it.
+template<bool PREV_LOCAL>
+__device__ void deferred_state(const InstDesc &D, const InstState *__restrict__ st, int inst,
+                               const Defer &df, InstState &shS)
+{
+    __syncthreads();                               // every thread has read the previous shS
+    wg_copy_state(&shS, &st[inst]);
+    __syncthreads();
+    if(!df.pending || shS.phase >= PH_DONE) return;             // the same for every thread
+    const int flux = shS.flux_next;                // whether the previous step summed the flux
+    double q[3] = {0.0, 0.0, 0.0};
+    if(flux) {
+        I128 s[3] = {i128_zero(), i128_zero(), i128_zero()};
+        int bad = 0;
+        const int p0 = PREV_LOCAL ? D.lp0 : D.np0, p1 = PREV_LOCAL ? D.lp1 : D.np1;
+        for(int p = p0 + (int)threadIdx.x; p < p1; p += (int)blockDim.x) {
+            const Partial pp = df.prev[p];
+#pragma unroll
+            for(int k = 0; k < 3; k++)
+                s[k] = i128_add(s[k], i128_from_chunks(pp.c[k][0], pp.c[k][1], pp.c[k][2]));
+            bad |= pp.bad;
+        }
+        wg_sum_i128(s, bad);
+        if(threadIdx.x == 0) {
+            const InstCfg &c = df.cfg[inst];
+            for(int k = 0; k < 3; k++) q[k] = i128_to_double_scaled(s[k]) / c.denom[k];
+            if(bad) q[0] = q[1] = q[2] = __builtin_nan("");
+        }
+    }
+    if(threadIdx.x == 0) {
+        const InstCfg &c = df.cfg[inst];
+        advance_state(c, shS, q, flux);
+        plan_next(c, shS);
+    }
+    __syncthreads();
+}
+
+// phase, flux and velocity flags of the step for the item's instance, and
+// the deferred state handed on by the instance's first item
+struct StepFlags { int phase, flux, vslot; };
+
+// DF: the kernel variant with the deferred finish; the convergence update
+// it carries needs about twice the registers of the step itself, so the
+// kernels without it (every lattice too large for it) are separate variants
+template<bool DF, bool PREV_LOCAL>
+__device__ __forceinline__ StepFlags step_flags(const InstDesc &D, const InstState *__restrict__ st, int inst,
+                                               int it, int first_slot, const Defer &df, InstState &shS,
+                                               int &cur)
+{
+    StepFlags r;
+    if constexpr(!DF) {
+        r.phase = st[inst].phase;
+        r.flux = st[inst].flux_next;
+        r.vslot = st[inst].vel_next;
+        return r;
+    }
+    if(inst != cur) {
+        deferred_state<PREV_LOCAL>(D, st, inst, df, shS);
+        cur = inst;
+    }
+    r.phase = (int)__builtin_amdgcn_readfirstlane((uint32_t)shS.phase);
+    r.flux = (int)__builtin_amdgcn_readfirstlane((uint32_t)shS.flux_next);
+    r.vslot = (int)__builtin_amdgcn_readfirstlane((uint32_t)shS.vel_next);
+    if(df.slot0 + it == first_slot) wg_copy_state(&df.sout[inst], &shS);
+    return r;
+}
+
 template<class L, int P> struct Store;
 template<> struct Store<D3Q19, PREC_F64> { using T = double; };
 template<> struct Store<D3Q19, PREC_F32> { using T = float; };
@@ -265,11 +384,13 @@ template<class T, int N> __device__ __forceinline__ T &lane_of(typename VecT<T, 
     else return reinterpret_cast<T *>(&v)[c];
 }
 
-template<class L, int P, int CPT, bool NT, bool W>
+template<class L, int P, int CPT, bool NT, bool W, bool DF>
 __global__ void __launch_bounds__(256)
 k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
-        const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part)
+        const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part, Defer df)
 {
+    __shared__ InstState shS;
+    int cur = -1;
     // W: 64-bit element indices (arrays of 2^32 elements or more)
     auto at = [](int k, int64_t i) -> uint64_t {
         if constexpr(W) return pop_at64<L::Q>(k, (uint64_t)i);
@@ -280,10 +401,11 @@ k_local(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
     const ItemRange R = my_items(nwork);
     for(int it = R.begin; it < R.end; it += R.step) {
         const LocalWork w = work[it];
-        if(st[w.inst].phase >= PH_DONE) continue;
-        const int flux = st[w.inst].flux_next;
-        const int vslot = st[w.inst].vel_next;
         const InstDesc &D = insts[w.inst];
+        const StepFlags sf = step_flags<DF, false>(D, st, w.inst, it, D.lp0, df, shS, cur);
+        if(sf.phase >= PH_DONE) continue;
+        const int flux = sf.flux;
+        const int vslot = sf.vslot;
         gptr<T> f = as_global<T>(D.f);
         const int c0 = (int)threadIdx.x * CPT;
         const int nreal = w.ncell - c0;             // real cells of this thread (may be <= 0)
@@ -362,11 +484,13 @@ __device__ __forceinline__ uint32_t lane_select(uint64_t M, uint32_t a, uint32_t
     return r;
 }
 
-template<class L, int P, int AD>
+template<class L, int P, int AD, bool DF>
 __global__ void __launch_bounds__(512)
 k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
-      const NbrWork *__restrict__ work, int nwork, Partial *__restrict__ part)
+      const NbrWork *__restrict__ work, int nwork, Partial *__restrict__ part, Defer df)
 {
+    __shared__ InstState shS;
+    int cur = -1;
     using T = typename Store<L, P>::T;
     constexpr int Q = L::Q;
     constexpr int NK = Q - 1;
@@ -375,10 +499,11 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
     const ItemRange R = my_items(nwork);
     for(int it = R.begin; it < R.end; it += R.step) {
         const NbrWork w = work[it];
-        if(st[w.inst].phase >= PH_DONE) continue;
-        const int flux = st[w.inst].flux_next;
-        const int vslot = st[w.inst].vel_next;
         const InstDesc &D = insts[w.inst];
+        const StepFlags sf = step_flags<DF, true>(D, st, w.inst, it, D.np0, df, shS, cur);
+        if(sf.phase >= PH_DONE) continue;
+        const int flux = sf.flux;
+        const int vslot = sf.vslot;
         gptr<T> f = as_global<T>(D.f);
         const int ny = D.ny, nseg = D.nseg;
         cptr<SegInfo> seg = as_const<SegInfo>(D.seg);
@@ -453,21 +578,24 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
 // case), else from vector loads.  The same slots as k_nbr are read and
 // written, so the two forms give identical results.
 // ---------------------------------------------------------------------------
-template<class L, int P, int AD>
+template<class L, int P, int AD, bool DF>
 __global__ void __launch_bounds__(256)
 k_nbr_c(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
-        const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part)
+        const LocalWork *__restrict__ work, int nwork, Partial *__restrict__ part, Defer df)
 {
+    __shared__ InstState shS;
+    int cur = -1;
     using T = typename Store<L, P>::T;
     constexpr int Q = L::Q;
     constexpr int NK = Q - 1;
     const ItemRange R = my_items(nwork);
     for(int it = R.begin; it < R.end; it += R.step) {
         const LocalWork w = work[it];
-        if(st[w.inst].phase >= PH_DONE) continue;
-        const int flux = st[w.inst].flux_next;
-        const int vslot = st[w.inst].vel_next;
         const InstDesc &D = insts[w.inst];
+        const StepFlags sf = step_flags<DF, true>(D, st, w.inst, it, D.np0, df, shS, cur);
+        if(sf.phase >= PH_DONE) continue;
+        const int flux = sf.flux;
+        const int vslot = sf.vslot;
         gptr<T> f = as_global<T>(D.f);
         const int c = (int)threadIdx.x;
         const bool act = c < w.ncell;
@@ -678,49 +806,52 @@ static int grid_for(int nwork, int blocks_per_cu)
 
 template<class L, int P, int CPT, bool NT>
 static void launch_local(const Tuning &tu, bool wide, const InstDesc *insts, const InstState *st,
-                         const LocalWork *lw, int nwork, Partial *part, hipStream_t s)
+                         const LocalWork *lw, int nwork, Partial *part, const Defer &df, hipStream_t s)
 {
     const dim3 g(grid_for(nwork, tu.local_bpc)), b(256);
-    if(wide) hipLaunchKernelGGL((k_local<L, P, CPT, NT, true>), g, b, 0, s, insts, st, lw, nwork, part);
-    else     hipLaunchKernelGGL((k_local<L, P, CPT, NT, false>), g, b, 0, s, insts, st, lw, nwork, part);
+    // the deferred finish runs on small lattices only (32-bit indices)
+    if(df.cfg) hipLaunchKernelGGL((k_local<L, P, CPT, NT, false, true>), g, b, 0, s, insts, st, lw, nwork, part, df);
+    else if(wide) hipLaunchKernelGGL((k_local<L, P, CPT, NT, true, false>), g, b, 0, s, insts, st, lw, nwork, part, df);
+    else     hipLaunchKernelGGL((k_local<L, P, CPT, NT, false, false>), g, b, 0, s, insts, st, lw, nwork, part, df);
 }
 
-template<class L, int P, int AD>
+template<class L, int P, int AD, bool DF>
 static void launch_nbr(const Tuning &tu, const InstDesc *insts, const InstState *st, const void *work,
-                       int nwork, Partial *part, hipStream_t s)
+                       int nwork, Partial *part, const Defer &df, hipStream_t s)
 {
     // the neighbour kernel's scattered partial-line traffic needs the L2:
     // it never uses non-temporal accesses (measured 1.6-1.8x the bytes)
     if(tu.nbr_mode == 1) {
         const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(256);
-        hipLaunchKernelGGL((k_nbr_c<L, P, AD>), g, b, 0, s, insts, st, (const LocalWork *)work, nwork, part);
+        hipLaunchKernelGGL((k_nbr_c<L, P, AD, DF>), g, b, 0, s, insts, st, (const LocalWork *)work, nwork, part, df);
     } else {
         const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
-        hipLaunchKernelGGL((k_nbr<L, P, AD>), g, b, 0, s, insts, st, (const NbrWork *)work, nwork, part);
+        hipLaunchKernelGGL((k_nbr<L, P, AD, DF>), g, b, 0, s, insts, st, (const NbrWork *)work, nwork, part, df);
     }
 }
 
 template<class L, int P>
 static void launch_step(bool local, const Tuning &tu, int ad, const InstDesc *insts, const InstState *st,
-                        const void *work, int nwork, Partial *part, hipStream_t s)
+                        const void *work, int nwork, Partial *part, const Defer &df, hipStream_t s)
 {
     if(nwork <= 0) return;
     if(local) {
         const LocalWork *lw = (const LocalWork *)work;
         const bool wide = ad == AD_E64;
         if(tu.nontemporal) {
-            if(tu.local_cpt == 4)      launch_local<L, P, 4, true>(tu, wide, insts, st, lw, nwork, part, s);
-            else if(tu.local_cpt == 2) launch_local<L, P, 2, true>(tu, wide, insts, st, lw, nwork, part, s);
-            else                       launch_local<L, P, 1, true>(tu, wide, insts, st, lw, nwork, part, s);
+            if(tu.local_cpt == 4)      launch_local<L, P, 4, true>(tu, wide, insts, st, lw, nwork, part, df, s);
+            else if(tu.local_cpt == 2) launch_local<L, P, 2, true>(tu, wide, insts, st, lw, nwork, part, df, s);
+            else                       launch_local<L, P, 1, true>(tu, wide, insts, st, lw, nwork, part, df, s);
         } else {
-            if(tu.local_cpt == 4)      launch_local<L, P, 4, false>(tu, wide, insts, st, lw, nwork, part, s);
-            else if(tu.local_cpt == 2) launch_local<L, P, 2, false>(tu, wide, insts, st, lw, nwork, part, s);
-            else                       launch_local<L, P, 1, false>(tu, wide, insts, st, lw, nwork, part, s);
+            if(tu.local_cpt == 4)      launch_local<L, P, 4, false>(tu, wide, insts, st, lw, nwork, part, df, s);
+            else if(tu.local_cpt == 2) launch_local<L, P, 2, false>(tu, wide, insts, st, lw, nwork, part, df, s);
+            else                       launch_local<L, P, 1, false>(tu, wide, insts, st, lw, nwork, part, df, s);
         }
     } else {
-        if(ad == AD_B32)      launch_nbr<L, P, AD_B32>(tu, insts, st, work, nwork, part, s);
-        else if(ad == AD_E32) launch_nbr<L, P, AD_E32>(tu, insts, st, work, nwork, part, s);
-        else                  launch_nbr<L, P, AD_E64>(tu, insts, st, work, nwork, part, s);
+        if(df.cfg)            launch_nbr<L, P, AD_B32, true>(tu, insts, st, work, nwork, part, df, s);
+        else if(ad == AD_B32) launch_nbr<L, P, AD_B32, false>(tu, insts, st, work, nwork, part, df, s);
+        else if(ad == AD_E32) launch_nbr<L, P, AD_E32, false>(tu, insts, st, work, nwork, part, df, s);
+        else                  launch_nbr<L, P, AD_E64, false>(tu, insts, st, work, nwork, part, df, s);
     }
 }
 
@@ -788,15 +919,18 @@ void kern_rest(int lattice, int prec, const InstDesc &D, hipStream_t s)
 }
 
 void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, const InstDesc *insts,
-               const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s)
+               const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s,
+               const Defer *defer)
 {
+    Defer df = {};
+    if(defer) df = *defer;
     if(lattice == 3) {
-        if(prec == PREC_F64) launch_step<D3Q19, PREC_F64>(local, tu, ad, insts, st, work, nwork, part, s);
-        else                 launch_step<D3Q19, PREC_F32>(local, tu, ad, insts, st, work, nwork, part, s);
+        if(prec == PREC_F64) launch_step<D3Q19, PREC_F64>(local, tu, ad, insts, st, work, nwork, part, df, s);
+        else                 launch_step<D3Q19, PREC_F32>(local, tu, ad, insts, st, work, nwork, part, df, s);
     } else {
-        if(prec == PREC_F64)      launch_step<D2Q9, PREC_F64>(local, tu, ad, insts, st, work, nwork, part, s);
-        else if(prec == PREC_F32) launch_step<D2Q9, PREC_F32>(local, tu, ad, insts, st, work, nwork, part, s);
-        else                      launch_step<D2Q9, PREC_F32_REF2D>(local, tu, ad, insts, st, work, nwork, part, s);
+        if(prec == PREC_F64)      launch_step<D2Q9, PREC_F64>(local, tu, ad, insts, st, work, nwork, part, df, s);
+        else if(prec == PREC_F32) launch_step<D2Q9, PREC_F32>(local, tu, ad, insts, st, work, nwork, part, df, s);
+        else                      launch_step<D2Q9, PREC_F32_REF2D>(local, tu, ad, insts, st, work, nwork, part, df, s);
     }
 }
 

@@ -92,7 +92,7 @@ class BenchOut(C.Structure):
     _fields_ = [("n_fluid", C.c_int64), ("n_cells", C.c_int64), ("steps", C.c_int64),
                 ("total_ms", C.c_double), ("local_ms", C.c_double), ("nbr_ms", C.c_double),
                 ("monitor_ms", C.c_double), ("bytes_per_update", C.c_double),
-                ("eager_total_ms", C.c_double)]
+                ("eager_total_ms", C.c_double), ("deferred", C.c_int32), ("pad", C.c_int32)]
 
 
 class Tuning(C.Structure):

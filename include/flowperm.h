@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 7
+#define FP_ABI_VERSION 8
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -309,6 +309,11 @@ typedef struct fp_bench_out {
     double  eager_total_ms;   /* the same steps launched one by one with an */
                               /* event pair around every kernel (the pass   */
                               /* the per-kernel averages come from)         */
+    int32_t deferred;         /* 1: total_ms ran with the finish of each    */
+                              /* step deferred into the next step kernel    */
+                              /* (lattices of at most 1024 work items per   */
+                              /* step kernel), one finish launch per chunk  */
+    int32_t pad;
 } fp_bench_out;
 
 /* Launch shapes of the step kernels; the autotuner picks them per device.  A new
