@@ -381,17 +381,29 @@ def precision_study(ctx, seed, steps):
     return res
 
 
-def large_lattice(ctx, seed, steps, solve):
+def fixed_tuning(spec):
+    """launch shapes given as cpt,waves,nontemporal,local_bpc,nbr_bpc[,nbr_mode]"""
+    v = [int(x) for x in spec.split(",")]
+    c, w, n, lb, nb = v[:5]
+    return {"local_cpt": c, "nbr_waves": w, "nontemporal": n, "local_bpc": lb,
+            "nbr_bpc": nb, "nbr_mode": v[5] if len(v) > 5 else 0, "chunk_steps": 200, "fixed": True}
+
+
+def large_lattice(ctx, seed, steps, solve, tune=""):
     """The 512^3 low-porosity Gaussian-random-field lattice: stepping rate in
-    FP32 and, with solve, the full three-direction FP32 solve."""
+    FP32 and, with solve, the full three-direction FP32 solve.  tune: fixed
+    launch shapes instead of the autotuner (profiles see only those)."""
     from flowperm import solver, workloads
     t0 = time.time()
     m = workloads.large_grf(seed)
     log(f"512^3 GRF: porosity {1 - m.mean():.4f} ({time.time() - t0:.1f} s to generate)")
     from flowperm import autotune
-    TUNED["f32"] = autotune.tune(ctx, m, 3, 1, steps=10, use_cache=False)
-    log(f"autotune f32 (512^3): {({k: TUNED['f32'][k] for k in autotune.FIELDS})}")
-    log(f"  neighbour ms {TUNED['f32']['nbr_ms']}")
+    if tune:
+        TUNED["f32"] = fixed_tuning(tune)
+    else:
+        TUNED["f32"] = autotune.tune(ctx, m, 3, 1, steps=10, use_cache=False)
+        log(f"autotune f32 (512^3): {({k: TUNED['f32'][k] for k in autotune.FIELDS})}")
+        log(f"  neighbour ms {TUNED['f32']['nbr_ms']}")
     apply_tuning(ctx, "f32")
     o = bench_steps(ctx, m, 1, 10, steps)
     res = {"porosity": float(1 - m.mean()), "n_fluid": int((m == 0).sum()),
@@ -620,9 +632,10 @@ def main():
     if a.study:
         ctx = solver.Context()
         from flowperm import autotune
-        mask, _ = workload_mask(256, a.seed)
-        for prec, name in ((1, "f32"), (0, "f64")):
-            TUNED[name] = autotune.tune(ctx, mask, 3, prec, steps=40, use_cache=False)
+        if not (a.tune and a.study.startswith("large")):
+            mask, _ = workload_mask(256, a.seed)
+            for prec, name in ((1, "f32"), (0, "f64")):
+                TUNED[name] = autotune.tune(ctx, mask, 3, prec, steps=40, use_cache=False)
         steps = max(2, a.steps & ~1)
         if a.study == "precision":
             out = {"study": "precision", **precision_study(ctx, a.seed, steps)}
@@ -631,7 +644,8 @@ def main():
         elif a.study == "batched-full":
             out = {"study": "batched-full", **batched_full(ctx, 512, 24, a.seed, a.workers)}
         else:
-            out = {"study": "large", **large_lattice(ctx, a.seed, steps, a.study == "large-solve")}
+            out = {"study": "large", **large_lattice(ctx, a.seed, steps, a.study == "large-solve",
+                                                     a.tune)}
         ctx.close()
         print(json.dumps(out), flush=True)
         return
@@ -649,11 +663,7 @@ def main():
         if prec == 0 and a.no_fp64:
             continue
         if a.tune:
-            v = [int(x) for x in a.tune.split(",")]
-            c, w, n, lb, nb = v[:5]
-            tuned[name] = {"local_cpt": c, "nbr_waves": w, "nontemporal": n, "local_bpc": lb,
-                           "nbr_bpc": nb, "nbr_mode": v[5] if len(v) > 5 else 0,
-                           "chunk_steps": 200, "fixed": True}
+            tuned[name] = fixed_tuning(a.tune)
             continue
         rec = autotune.tune(ctx, mask, 3, prec, steps=40, use_cache=False)
         tuned[name] = rec

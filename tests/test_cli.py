@@ -177,7 +177,11 @@ def test_lbm3d_velocity_files(tmp_path):
 
 
 @pytest.mark.gpu
-def test_lbm2d_row_and_velocity_export(tmp_path):
+def test_lbm2d_row_and_velocity_export(tmp_path, oracle):
+    """the 2D executable on a GIF: its CSV row equals the in-process solve's
+    text, and its values those of the CPU oracle's whole 2D invocation (the
+    reference's float arithmetic, L0 = 4 * 64 / 256) to the printed digits;
+    the .dat velocity file holds the exported field"""
     from flowperm import masks, solver, structio
     m = masks.ellipses((64, 64), 0.6, seed=1)
     structio.save_gif2d(tmp_path / "m.gif", m)
@@ -190,6 +194,17 @@ def test_lbm2d_row_and_velocity_export(tmp_path):
     r = solver.solve_permeability2d(m, lag=500, min_steps=1000, max_steps=8000, avg_steps=1000,
                                     velocity=True, filename=str(tmp_path / "m.gif"))
     assert lines[1] + "\n" == solver.csv_row(r, solver.COLUMNS2D)
+    ours = dict(zip(lines[0].split(","), next(csv.reader([lines[1]]))))
+    _, T = oracle.solve_tensor2d(m, L0=1.0, lag=500, min_steps=1000, max_steps=8000, avg_steps=1000)
+    for col, want in (("steps_fx", T.run[0].steps), ("steps_fy", T.run[1].steps),
+                      ("conv_fx", T.run[0].converged), ("conv_fy", T.run[1].converged)):
+        assert int(ours[col]) == want, col
+    for col, want in (("k_xx", T.k_xx), ("k_yx", T.k_yx), ("k_xy", T.k_xy), ("k_yy", T.k_yy),
+                      ("qx_fx", T.run[0].qx), ("qy_fx", T.run[0].qy), ("qx_fy", T.run[1].qx),
+                      ("qy_fy", T.run[1].qy), ("fieldres_fx", T.run[0].fieldres),
+                      ("fieldres_fy", T.run[1].fieldres), ("anisotropy", T.aniso),
+                      ("theta_deg", T.theta)):
+        assert abs(float(ours[col]) - want) <= 5e-10 * abs(want), col
     txt = (tmp_path / "v.fx.dat").read_text().split()
     vals = np.array([float(x) for x in txt]).reshape(64, 64, 2)
     assert np.allclose(vals, r["velocity"][0], rtol=1e-5, atol=1e-12)
