@@ -280,6 +280,38 @@ def serving_sample(ctx, n, seed, workers, load=0.8):
     return res
 
 
+def small_lattice_steps(ctx, seed):
+    """per-step time of the reference's own lattice sizes solved one at a
+    time (graph replays): D2Q9 256^2 and D3Q19 64^3 / 80^3, and whether the
+    finish ran deferred into the step kernels, with the default launch shapes
+    (2 cells per thread, 4-row slabs: 64^3 then has 1024 neighbour items, the
+    most the deferred finish takes)"""
+    from flowperm import _lib, masks
+    res = {}
+    shapes = _lib.Tuning(2, 4, 200, 1, 0, 0, 0)
+    cases = [("d2q9_256", masks.ellipses((256, 256), 0.6, seed=seed + 3), 2,
+              [("f32", 1), ("f32ref", 2), ("f64", 0)]),
+             ("d3q19_64", masks.ellipsoids((64, 64, 64), 0.6, seed=seed + 4), 3, [("f32", 1), ("f64", 0)]),
+             ("d3q19_80", masks.ellipsoids((80, 80, 80), 0.6, seed=seed + 4), 3, [("f32", 1), ("f64", 0)])]
+    for name, m, lattice, precs in cases:
+        if lattice == 3:
+            nz, ny, nx = m.shape
+        else:
+            (ny, nx), nz = m.shape, 1
+        m = np.ascontiguousarray(m, np.uint8)
+        for pname, prec in precs:
+            out = _lib.BenchOut()
+            F = np.array([2.5e-7 * 40, 0.0, 0.0])
+            st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(m), nx, ny, nz, lattice, prec,
+                                        _lib.f64ptr(F), 200, 2000, C.byref(shapes), 0, C.byref(out))
+            if st != 0:
+                raise RuntimeError(f"fp_bench_steps ({name}) failed: {ctx.last_error()}")
+            res[f"{name}_{pname}"] = {"us_per_step": round(out.total_ms / out.steps * 1e3, 3),
+                                      "n_fluid": int(out.n_fluid), "deferred_finish": int(out.deferred)}
+    log(f"small lattices (us per step): { {k: v['us_per_step'] for k, v in res.items()} }")
+    return res
+
+
 def k2_matrix(row):
     return np.array([[row["k_xx"], row["k_xy"]], [row["k_yx"], row["k_yy"]]])
 
@@ -717,6 +749,7 @@ def main():
             "local_ms": o64.local_ms, "neighbour_ms": o64.nbr_ms,
             "step_kernel_GBs": o64.n_fluid * o64.bytes_per_update / (k64 / 1e3) / 1e9}
         log(f"FP64: {extra['fp64']['mlups']:.1f} MLUPS, {extra['fp64']['step_kernel_GBs']:.0f} GB/s")
+    extra["small_lattices"] = small_lattice_steps(ctx, a.seed)
     if a.batched > 0:
         fp64_gbs = extra.get("fp64", {}).get("step_kernel_GBs")
         extra["batched"] = batched_sample(ctx, a.batched, min(a.batched, a.batched_fp64), a.seed,
