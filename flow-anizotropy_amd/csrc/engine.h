@@ -161,6 +161,7 @@ private:
     // by its finish, or with the deferred finish and one finish at the end
     void enqueue_steps(int n);
     bool wants_rebuild() const;
+    bool choose_defer() const;
     int64_t work_items(const Job &j) const;
     void release_graph();
     void free_job(Job *j);
@@ -176,7 +177,9 @@ private:
     InstCfg *d_cfg_ = nullptr;
     InstState *d_state_ = nullptr;
     InstState *d_state2_ = nullptr;          // the other state array of the deferred finish
-    bool defer_ = false;
+    bool defer_ = false;                     // the mode of the last chunk
+    bool defer_ok_ = false;                  // every instance within DEFER_MAX_PARTIALS
+    bool defer_small_ = false;               // ... and within REDUCE_SPAN
     LocalWork *d_lwork_ = nullptr;
     NbrWork *d_nwork_ = nullptr;
     LocalWork *d_cwork_ = nullptr;           // compact neighbour-step items (mode 1)
@@ -196,8 +199,8 @@ private:
     int64_t work_total_ = 0, work_dead_ = 0;     // launched items, items of retired jobs
     size_t cap_inst_ = 0, cap_lwork_ = 0, cap_nwork_ = 0, cap_lpart_ = 0, cap_npart_ = 0;
     Tuning tune_;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
+    hipGraph_t graph_[2] = {nullptr, nullptr};       // chunk graphs: finish per step, deferred
+    hipGraphExec_t exec_[2] = {nullptr, nullptr};
     BatchStats stats_;
 };
 

@@ -468,8 +468,10 @@ Batch::~Batch()
 
 void Batch::release_graph()
 {
-    if(exec_) { (void)hipGraphExecDestroy(exec_); exec_ = nullptr; }
-    if(graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    for(int m = 0; m < 2; m++) {
+        if(exec_[m]) { (void)hipGraphExecDestroy(exec_[m]); exec_[m] = nullptr; }
+        if(graph_[m]) { (void)hipGraphDestroy(graph_[m]); graph_[m] = nullptr; }
+    }
 }
 
 void Batch::free_job(Job *j)
@@ -809,13 +811,19 @@ std::string Batch::rebuild()
     for(int i = 0; i < n_inst_; i++) { cfg[i] = jobs_[i]->cfg; state[i] = jobs_[i]->state; }
     n_lred_ = (int)lr.size();
     n_nred_ = (int)nr.size();
-    // the deferred finish needs every instance's partials of a step kernel
-    // in one reduction item (so that one workgroup can sum them);
-    // FLOWPERM_DEFER_FINISH=0 turns it off
+    // the deferred finish has every workgroup of an instance sum the
+    // instance's partials of the previous step: up to REDUCE_SPAN of them on
+    // any step, up to DEFER_MAX_PARTIALS when the flux is summed on few steps
+    // (choose_defer); FLOWPERM_DEFER_FINISH=0 turns it off
     const char *dv = getenv("FLOWPERM_DEFER_FINISH");
-    defer_ = !(dv && dv[0] == '0') && ad_ == AD_B32;
-    for(int i = 0; i < n_inst_ && defer_; i++)
-        if(desc[i].rl1 - desc[i].rl0 != 1 || desc[i].rn1 - desc[i].rn0 != 1) defer_ = false;
+    defer_ok_ = !(dv && dv[0] == '0') && ad_ == AD_B32;
+    defer_small_ = defer_ok_;
+    for(int i = 0; i < n_inst_; i++) {
+        const int32_t nl = desc[i].lp1 - desc[i].lp0, nn = desc[i].np1 - desc[i].np0;
+        if(nl > DEFER_MAX_PARTIALS || nn > DEFER_MAX_PARTIALS) defer_ok_ = false;
+        if(nl > REDUCE_SPAN || nn > REDUCE_SPAN) defer_small_ = false;
+    }
+    defer_small_ = defer_small_ && defer_ok_;
     if(lw.size() > INT32_MAX || nw.size() > INT32_MAX || cw.size() > INT32_MAX) return "batch too large";
     n_lwork_ = (int)lw.size();
     n_nwork_ = (int)nw.size();
@@ -873,6 +881,24 @@ bool Batch::wants_rebuild() const
     return dirty_ || !pending_.empty() || (work_total_ > 0 && 4 * work_dead_ > work_total_);
 }
 
+// Deferred finish for the next chunk?  Always when every instance has at most
+// REDUCE_SPAN partials per step kernel; with up to DEFER_MAX_PARTIALS only
+// while no live instance sums its flux on most steps (a 3D averaging phase,
+// or checks every few steps), where every workgroup summing thousands of
+// partials after every step would cost more than the finish launch it saves.
+bool Batch::choose_defer() const
+{
+    if(!defer_ok_) return false;
+    if(defer_small_) return true;
+    for(const Job *j : jobs_) {
+        if(!j || j->done) continue;
+        const InstCfg &c = j->cfg;
+        const bool dense = c.kind == 3 ? (j->state.phase == PH_AVG || c.lag < 16) : c.avg_sample < 16;
+        if(dense) return false;
+    }
+    return true;
+}
+
 std::string Batch::run_chunk()
 {
     if(wants_rebuild()) {
@@ -882,18 +908,20 @@ std::string Batch::run_chunk()
     if(n_inst_ == 0) return "";
     const double t0 = now_seconds();
     hipError_t e;
+    defer_ = choose_defer();
     auto launch_all = [&]() { enqueue_steps(chunk_); };
     if(use_graph_) {
-        if(!exec_) {
+        const int m = defer_ ? 1 : 0;
+        if(!exec_[m]) {
             e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
             if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
             launch_all();
-            e = hipStreamEndCapture(stream_, &graph_);
-            if(e != hipSuccess) { graph_ = nullptr; return hip_err("hipStreamEndCapture", e); }
-            e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
-            if(e != hipSuccess) { exec_ = nullptr; return hip_err("hipGraphInstantiate", e); }
+            e = hipStreamEndCapture(stream_, &graph_[m]);
+            if(e != hipSuccess) { graph_[m] = nullptr; return hip_err("hipStreamEndCapture", e); }
+            e = hipGraphInstantiate(&exec_[m], graph_[m], nullptr, nullptr, 0);
+            if(e != hipSuccess) { exec_[m] = nullptr; return hip_err("hipGraphInstantiate", e); }
         }
-        e = hipGraphLaunch(exec_, stream_);
+        e = hipGraphLaunch(exec_[m], stream_);
         if(e != hipSuccess) return hip_err("hipGraphLaunch", e);
     } else {
         launch_all();
@@ -1049,6 +1077,7 @@ std::string Batch::graph_steps(int steps, float &ms)
     if(n_inst_ == 0) return "nothing to step";
     steps &= ~1;
     if(steps < 2) return "";
+    defer_ = choose_defer();
     const int c = std::min(steps, 200) & ~1;         // every replay starts at an even step
     const int reps = steps / c, rest = steps - reps * c;
     hipGraph_t g[2] = {nullptr, nullptr};

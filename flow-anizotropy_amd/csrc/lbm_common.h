@@ -540,6 +540,10 @@ struct Partial {
 // second level: up to REDUCE_SPAN step-kernel partials of one instance
 // folded into 128-bit integers by one workgroup
 constexpr int REDUCE_SPAN = 1024;
+// the deferred finish (struct Defer) takes instances of up to this many
+// partials per step kernel (every workgroup of such an instance sums them on
+// the steps after a flux step)
+constexpr int DEFER_MAX_PARTIALS = 4096;
 struct RPartial {
     I128    s[3];
     int32_t bad;
@@ -552,7 +556,7 @@ struct RedWork {
 };
 
 // The deferred finish.  When every instance of a batch has at most
-// REDUCE_SPAN partials per step kernel, a step kernel applies the previous
+// DEFER_MAX_PARTIALS partials per step kernel, a step kernel applies the previous
 // step's finish itself instead of a finish launch after every step: each
 // workgroup reduces its instance's partials of the previous step (on flux
 // steps) and runs the convergence update, all workgroups of an instance

@@ -1,8 +1,9 @@
 """
 The deferred finish: on lattices with at most 1024 work items per step kernel
-each step kernel applies the previous step's flux reduction and convergence
-update itself (every workgroup for its own instance), and a chunk ends with a
-single finish launch, instead of a finish launch after every step.  It must
+(up to 4096 while the flux is summed on few steps) each step kernel applies
+the previous step's flux reduction and convergence update itself (every
+workgroup for its own instance), and a chunk ends with a single finish launch,
+instead of a finish launch after every step.  It must
 not change a single bit: every result equals the one with a finish launch
 after every step (FLOWPERM_DEFER_FINISH=0), for both neighbour-step forms, a
 batch mixing them, 3D and 2D in every precision, with the velocity fields and
@@ -63,14 +64,18 @@ def _bench(ctx, mask, lattice, prec):
 
 
 def test_the_deferred_path_runs_where_it_applies(gpu_ctx):
+    """up to 1024 partials per step kernel: always; up to 4096: while the
+    flux is summed on few steps (the bench's steps: a check every 2000);
+    more: never"""
     from flowperm import masks
     small3 = masks.spheres((24, 20, 28), 0.6, 2, 4, seed=3)
     small2 = masks.ellipses((256, 256), 0.6, seed=3)
-    large3 = masks.spheres((128, 128, 128), 0.6, 3, 6, seed=3)   # 1,250+ local items
+    medium3 = masks.spheres((80, 80, 80), 0.6, 3, 6, seed=3)     # ~600 local, 3200 neighbour items
+    large3 = masks.spheres((160, 160, 160), 0.6, 3, 6, seed=3)   # ~4800 local items
     print("deferred flags", flush=True)
-    a, b = _both(lambda: (_bench(gpu_ctx, small3, 3, 1).deferred, _bench(gpu_ctx, small2, 2, 2).deferred,
-                          _bench(gpu_ctx, large3, 3, 1).deferred))
-    assert a == (1, 1, 0) and b == (0, 0, 0)
+    a, b = _both(lambda: tuple(_bench(gpu_ctx, m, lat, 1).deferred
+                               for m, lat in ((small3, 3), (small2, 2), (medium3, 3), (large3, 3))))
+    assert a == (1, 1, 1, 0) and b == (0, 0, 0, 0)
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2], ids=["row_segments", "compact", "per_structure"])
@@ -119,6 +124,23 @@ def test_deferred_finish_bitwise_mixed_batch():
     for ra, rb in zip(a, b):
         for k in KEYS3:
             assert ra[k] == rb[k], k
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_deferred_finish_bitwise_medium_3d(gpu_ctx, precision):
+    """an 80^3 lattice: more than 1024 neighbour-step partials, so the chunks
+    of its averaging phase (flux on every step) run with a finish launch per
+    step and the first phase's deferred; the whole solve equals the one with
+    a finish launch after every step"""
+    from flowperm import masks, solver
+    m = masks.ellipsoids((80, 80, 80), 0.6, seed=4)
+    cfg = dict(ff=40, eps=1e-6, lag=200, min_steps=400, avg_steps=300, max_steps=3000,
+               precision=precision)
+    print(f"80^3 {precision}", flush=True)
+    a, b = _both(lambda: solver.solve_permeability3d(m, ctx=gpu_ctx, velocity=True, **cfg))
+    for k in KEYS3:
+        assert a[k] == b[k], k
+    assert np.array_equal(a["velocity"], b["velocity"])
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32", "f32ref"])
