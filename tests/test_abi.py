@@ -185,3 +185,62 @@ def test_neighbour_link_tables_match_a_dense_lookup(lib):
                 st = lib.fp_verify_links(_lib.u8ptr(np.ascontiguousarray(m)), dim, nx, ny, nz,
                                          C.byref(bad))
                 assert st == 0 and bad.value == 0, (s, dim, por, bad.value)
+
+
+C_CALLER = r"""
+#include <stdio.h>
+#include <stdlib.h>
+#include "flowperm.h"
+int main(void)
+{
+    printf("abi %d\n", fp_abi_version());
+    fp_config3d c;
+    fp_config3d_default(&c);
+    printf("defaults %g %g %lld %d %lld %lld %lld\n", c.ff, c.eps, (long long)c.lag, c.hold,
+           (long long)c.min_steps, (long long)c.max_steps, (long long)c.avg_steps);
+    printf("exit %d %d %d %d\n", fp_exit_code(FP_OK), fp_exit_code(FP_NOT_CONVERGED),
+           fp_exit_code(FP_ERR_NON_BINARY), fp_exit_code(FP_ERR_NO_PORE));
+    uint8_t m[8] = {0, 1, 1, 0, 1, 1, 1, 0};
+    int64_t np = -1;
+    int ms = fp_check_mask3d(m, 2, 2, 2, &np);
+    printf("mask %d %lld\n", ms, (long long)np);
+    m[3] = 7;
+    printf("mask %d\n", fp_check_mask3d(m, 2, 2, 2, &np));
+    /* a constant flux under lag 100, hold 2, min 200, avg 100: checks at 100
+     * (recorded), 200 and 300 (hold 1, 2), then 100 steps of averaging */
+    c.lag = 100; c.min_steps = 200; c.max_steps = 2000; c.avg_steps = 100; c.eps = 1e-6;
+    enum { N = 1000 };
+    double *q = malloc(sizeof(double) * 3 * N);
+    for(int s = 0; s < N; s++) { q[3*s] = 1.0 / 1024; q[3*s+1] = -1.0 / 2048; q[3*s+2] = 0.0; }
+    fp_solve_out o;
+    int st = fp_replay_convergence3d(&c, q, N, &o);
+    printf("replay %d %lld %d %.17g %lld %lld %d\n", st, (long long)o.steps, o.converged, o.q[0],
+           (long long)o.check_step, (long long)o.checks, o.check_hold);
+    free(q);
+    return 0;
+}
+"""
+
+
+def test_a_c_caller_links_and_calls_the_host_entry_points(tmp_path):
+    """the boundary as a C program sees it: the header compiles as C99, the
+    program links against libflowperm.so and the entry points that need no
+    GPU (ABI version, reference defaults, exit codes, the mask check, the
+    convergence rules on a planted flux) return what the reference defines"""
+    from flowperm import _lib
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    exe = tmp_path / "caller"
+    libdir = _lib.LIB_PATH.parent
+    inc = REPO / "include"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(inc), "-o", str(exe), str(src),
+                    "-L", str(libdir), "-lflowperm", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=120).stdout
+    lines = out.splitlines()
+    version = re.search(r"FP_ABI_VERSION (\d+)", (inc / "flowperm.h").read_text()).group(1)
+    assert lines[0] == f"abi {version}"
+    assert lines[1] == "defaults 1 1e-05 2000 2 4000 200000 2000"     # LMB3d/lbm3d-perm.cpp:118-130
+    assert lines[2] == "exit 0 3 2 2"
+    assert lines[3] == "mask 0 3"
+    assert lines[4].startswith("mask ") and lines[4] != "mask 0"
+    assert lines[5] == f"replay 0 400 1 {1.0 / 1024!r} 300 3 2"
