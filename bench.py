@@ -649,9 +649,11 @@ def main():
     ap.add_argument("--serving", type=int, default=40, help="requests in the serving sample (0 = skip)")
     ap.add_argument("--workers", type=int, default=8, help="processes generating structures")
     ap.add_argument("--study", default="", choices=["", "precision", "large", "large-solve",
-                                                     "batched-full", "small-batched"],
-                    help="instead of the default line: the 256^3 / 1024^2 precision study, or "
-                         "the 512^3 GRF lattice (stepping rate; large-solve adds the full solve)")
+                                                     "batched-full", "small-batched", "small-steps"],
+                    help="instead of the default line: the 256^3 / 1024^2 precision study, "
+                         "the 512^3 GRF lattice (stepping rate; large-solve adds the full solve), "
+                         "the whole batched set, the reference's dataset sizes batched, or the "
+                         "per-step times of the small lattices (small-steps)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -664,6 +666,11 @@ def main():
     if a.study:
         ctx = solver.Context()
         from flowperm import autotune
+        if a.study == "small-steps":             # per-step times of the small lattices only
+            out = {"study": "small-steps", **small_lattice_steps(ctx, a.seed)}
+            ctx.close()
+            print(json.dumps(out), flush=True)
+            return
         if not (a.tune and a.study.startswith("large")):
             mask, _ = workload_mask(256, a.seed)
             for prec, name in ((1, "f32"), (0, "f64")):
