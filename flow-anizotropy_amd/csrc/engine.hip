@@ -858,6 +858,9 @@ std::string Batch::rebuild()
     e = hipMemcpy(d_inst_, desc.data(), desc.size() * sizeof(InstDesc), hipMemcpyHostToDevice);
     if(e == hipSuccess) e = hipMemcpy(d_cfg_, cfg.data(), cfg.size() * sizeof(InstCfg), hipMemcpyHostToDevice);
     if(e == hipSuccess) e = hipMemcpy(d_state_, state.data(), state.size() * sizeof(InstState), hipMemcpyHostToDevice);
+    // the deferred finish's second state array is written by every step before
+    // it is read; it starts as a copy so that no kernel can ever see garbage
+    if(e == hipSuccess) e = hipMemcpy(d_state2_, state.data(), state.size() * sizeof(InstState), hipMemcpyHostToDevice);
     if(e == hipSuccess && !lw.empty()) e = hipMemcpy(d_lwork_, lw.data(), lw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !nw.empty()) e = hipMemcpy(d_nwork_, nw.data(), nw.size() * sizeof(NbrWork), hipMemcpyHostToDevice);
     if(e == hipSuccess && !cw.empty()) e = hipMemcpy(d_cwork_, cw.data(), cw.size() * sizeof(LocalWork), hipMemcpyHostToDevice);

@@ -310,6 +310,7 @@ __device__ __forceinline__ void node(typename Store<L, P>::T *v, const InstDesc 
 template<class L>
 __device__ __forceinline__ void write_vel(const InstDesc &D, int slot, int64_t i, const double *u)
 {
+    if(slot != 1 && slot != 2) return;             // the two snapshot slots only
     if(!D.vel[slot - 1]) return;
     gptr<float> v = as_global<float>(D.vel[slot - 1]);
 #pragma unroll
