@@ -639,6 +639,7 @@ def main():
                     help="workload mask (open = all fluid, a diagnostic)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
+    ap.add_argument("--no-small", action="store_true", help="skip the small-lattice step times")
     ap.add_argument("--tune", default="", help="fixed launch shapes "
                     "'cpt,waves,nt,local_bpc,nbr_bpc[,nbr_mode]' for both precisions instead of "
                     "running the autotuner")
@@ -756,7 +757,8 @@ def main():
             "local_ms": o64.local_ms, "neighbour_ms": o64.nbr_ms,
             "step_kernel_GBs": o64.n_fluid * o64.bytes_per_update / (k64 / 1e3) / 1e9}
         log(f"FP64: {extra['fp64']['mlups']:.1f} MLUPS, {extra['fp64']['step_kernel_GBs']:.0f} GB/s")
-    extra["small_lattices"] = small_lattice_steps(ctx, a.seed)
+    if not a.no_small:
+        extra["small_lattices"] = small_lattice_steps(ctx, a.seed)
     if a.batched > 0:
         fp64_gbs = extra.get("fp64", {}).get("step_kernel_GBs")
         extra["batched"] = batched_sample(ctx, a.batched, min(a.batched, a.batched_fp64), a.seed,
