@@ -61,6 +61,29 @@ def test_gif_round_trip(tmp_path):
     assert np.array_equal(structio.load_structure2d(tmp_path / "m.gif"), m)
 
 
+def test_gif_red_channel_decides(tmp_path):
+    """read_from_gif (LMB2d/lbm.cpp:412-426) tests only the decoded red channel:
+    a palette colour with red 0 is solid whatever its green and blue (a
+    luminance or any-channel reading would call it pore), any red > 0 is pore,
+    and a transparent palette index still decodes to its palette colour"""
+    pytest.importorskip("PIL")
+    from PIL import Image
+    from flowperm import structio
+    palette = [(0, 0, 0), (1, 0, 0), (0, 255, 0), (0, 0, 255), (0, 255, 255), (255, 255, 255),
+               (200, 0, 0), (0, 1, 1)]
+    pore = np.array([r > 0 for r, g, b in palette])
+    rng = np.random.default_rng(11)
+    idx = rng.integers(0, len(palette), size=(9, 13)).astype(np.uint8)
+    idx[0, :len(palette)] = np.arange(len(palette))               # every colour appears
+    im = Image.fromarray(idx, mode="P")
+    flat = [c for rgb in palette for c in rgb]
+    im.putpalette(flat + [0] * (768 - len(flat)))
+    im.save(tmp_path / "p.gif", transparency=2)                    # (0, 255, 0) transparent
+    m = structio.load_structure2d(tmp_path / "p.gif")
+    assert m.shape == (9, 13)
+    assert np.array_equal(m, (~pore[idx]).astype(np.uint8))
+
+
 def test_atoi_semantics():
     """the C library's atoi on one extracted token (LMB2d/lbm.cpp:443, :464)"""
     from flowperm.structio import atoi
