@@ -549,6 +549,27 @@ int fp_verify_batch_tables(const uint8_t *const *solids, const int32_t *dims, in
     return FP_OK;
 }
 
+int fp_verify_finish_plan(const fp_config3d *c3, const fp_config2d *c2, int n, const double *q,
+                          int64_t nsteps, const int64_t *start, int chunk, int replays, int plant,
+                          int64_t *problems, int64_t *finishes)
+{
+    if((!c3 == !c2) || n <= 0 || !q || nsteps < 0 || !start || chunk < 2 || (chunk & 1) ||
+       replays < 1 || !problems)
+        return FP_ERR_ARG;
+    std::vector<InstCfg> cfg(n);
+    for(int i = 0; i < n; i++) {
+        if(c3) {
+            if(!cfg3_ok(&c3[i])) return FP_ERR_ARG;
+            cfg[i] = cfg3(c3[i], 1, (i & 1) != 0);          // every other one writes its velocity
+        } else {
+            if(!cfg2_ok(&c2[i])) return FP_ERR_ARG;
+            cfg[i] = cfg2(c2[i], 1, 1);
+        }
+    }
+    *problems = verify_finish_plan(cfg, q, nsteps, start, chunk, replays, plant, finishes);
+    return FP_OK;
+}
+
 int fp_check_mask3d(const uint8_t *solid, int nx, int ny, int nz, int64_t *n_pore)
 {
     if(nx <= 0 || ny <= 0 || nz <= 0) return FP_ERR_BAD_DIMS;

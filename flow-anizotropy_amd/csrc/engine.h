@@ -23,10 +23,11 @@ void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, cons
 // 8 KiB, 2 the same with non-temporal loads and stores
 constexpr int N_COPY_KINDS = 3;
 void kern_copy(int kind, const void *a, void *b, int64_t n16, hipStream_t s);
-// after a step: exact flux sums and the convergence update of every instance
+// after a step: exact flux sums and the convergence update of every instance;
+// adv > 1 also counts the adv - 1 quiet steps before it (quiet_run_step)
 void kern_finish(int nwork, const RedWork *work, const InstDesc *insts, const InstCfg *cfgs,
                  InstState *st, const Partial *part, RPartial *red, unsigned int *counter,
-                 bool local, hipStream_t s);
+                 bool local, hipStream_t s, int adv = 1);
 
 // One voxel structure resident on the device: the segment table.
 struct Geometry {
@@ -156,10 +157,14 @@ public:
 private:
     std::string rebuild();
     void launch_step(bool local, const Defer *df = nullptr);
-    void launch_finish(bool local);
-    // the kernels of n steps (even) from an even step count, each followed
-    // by its finish, or with the deferred finish and one finish at the end
-    void enqueue_steps(int n);
+    void launch_finish(bool local, int adv = 1);
+    // the kernels of n steps (even) from an even step count: each followed by
+    // its finish, or by a finish only where adv[t] > 0 (plan_finishes), or
+    // with the deferred finish and one finish at the end
+    void enqueue_steps(int n, const std::vector<int> *adv = nullptr);
+    // the finish plan of the next n steps, run `replays` times back to back
+    // after `skip` steps, from the jobs' current states
+    void plan_chunk(int64_t skip, int n, int replays, std::vector<int> &adv) const;
     bool wants_rebuild() const;
     bool choose_defer() const;
     int64_t work_items(const Job &j) const;
@@ -201,6 +206,15 @@ private:
     Tuning tune_;
     hipGraph_t graph_[2] = {nullptr, nullptr};       // chunk graphs: finish per step, deferred
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
+    // chunk graphs of the finish plans met since the last rebuild (the plan
+    // of a stable job set repeats with the lag; most recently used last)
+    struct PlanGraph {
+        std::vector<int> adv;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+    };
+    std::vector<PlanGraph> plan_graphs_;
+    bool sparse_ok_ = true;                  // finish plans in use (FLOWPERM_SPARSE_FINISH)
     BatchStats stats_;
 };
 
@@ -219,6 +233,30 @@ int64_t check_tables(const BatchTables &t, std::string *first);
 std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,
                                 const std::vector<int> &dims, int lattice, int prec,
                                 const Tuning &tu, int64_t *bad);
+
+// Which steps of a chunk of n steps must end with a finish, when the chunk
+// runs `replays` times back to back after `skip` steps from the states st
+// (the same plan for every run: one captured graph).  adv[t] > 0: the finish
+// after step t advances the state by adv[t] steps -- the quiet steps since the
+// previous finish and step t (advance_state_n); adv[t] = 0: no finish.  A
+// finish follows every step that is not quiet for some live instance
+// (quiet_run_step) and every step before one (its finish plans that step's
+// outputs), the last step of the chunk, and every step from the first at
+// which an instance's first phase may end (max_steps, or the check at which
+// hold can first reach c.hold).  Instances in their averaging phase take a
+// finish after every step; retired ones (phase >= PH_DONE) none.
+void plan_finishes(const InstCfg *cfg, const InstState *st, int n_inst, int64_t skip, int n,
+                   int replays, std::vector<int> &adv);
+// host only: a batch of n instances with the flux sequences q (nsteps x 3 per
+// instance), joining at the batch steps start[i], stepped both with a finish
+// after every step and with the finish plans of chunks of `chunk` steps run
+// `replays` times per plan; counts in *problems every step whose outputs
+// (phase, flux_next, vel_next) and every finish whose state differ.  plant = 1
+// replaces each plan by one finish at the chunk's end (a wrong plan the check
+// must catch).
+int64_t verify_finish_plan(const std::vector<InstCfg> &cfg, const double *q, int64_t nsteps,
+                           const int64_t *start, int chunk, int replays, int plant,
+                           int64_t *finishes);
 
 // the D3Q19 FP32 per-direction force terms of InstDesc
 void f32_force_terms(const double F[3], float g45[19], float h15[19]);

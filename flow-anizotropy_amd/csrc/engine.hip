@@ -472,6 +472,11 @@ void Batch::release_graph()
         if(exec_[m]) { (void)hipGraphExecDestroy(exec_[m]); exec_[m] = nullptr; }
         if(graph_[m]) { (void)hipGraphDestroy(graph_[m]); graph_[m] = nullptr; }
     }
+    for(PlanGraph &p : plan_graphs_) {
+        if(p.x) (void)hipGraphExecDestroy(p.x);
+        if(p.g) (void)hipGraphDestroy(p.g);
+    }
+    plan_graphs_.clear();
 }
 
 void Batch::free_job(Job *j)
@@ -815,6 +820,8 @@ std::string Batch::rebuild()
     // instance's partials of the previous step: up to REDUCE_SPAN of them on
     // any step, up to DEFER_MAX_PARTIALS when the flux is summed on few steps
     // (choose_defer); FLOWPERM_DEFER_FINISH=0 turns it off
+    const char *sv = getenv("FLOWPERM_SPARSE_FINISH");      // 0: a finish after every step
+    sparse_ok_ = !(sv && sv[0] == '0');
     const char *dv = getenv("FLOWPERM_DEFER_FINISH");
     defer_ok_ = !(dv && dv[0] == '0') && ad_ == AD_B32;
     defer_small_ = defer_ok_;
@@ -912,19 +919,52 @@ std::string Batch::run_chunk()
     const double t0 = now_seconds();
     hipError_t e;
     defer_ = choose_defer();
-    auto launch_all = [&]() { enqueue_steps(chunk_); };
+    std::vector<int> adv;
+    const bool planned = !defer_ && sparse_ok_;
+    if(planned) plan_chunk(0, chunk_, 1, adv);
+    auto launch_all = [&]() { enqueue_steps(chunk_, planned ? &adv : nullptr); };
     if(use_graph_) {
-        const int m = defer_ ? 1 : 0;
-        if(!exec_[m]) {
-            e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
-            if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
-            launch_all();
-            e = hipStreamEndCapture(stream_, &graph_[m]);
-            if(e != hipSuccess) { graph_[m] = nullptr; return hip_err("hipStreamEndCapture", e); }
-            e = hipGraphInstantiate(&exec_[m], graph_[m], nullptr, nullptr, 0);
-            if(e != hipSuccess) { exec_[m] = nullptr; return hip_err("hipGraphInstantiate", e); }
+        hipGraphExec_t x = nullptr;
+        if(planned) {
+            // the graph of this plan: cached, most recently used last
+            for(size_t i = 0; i < plan_graphs_.size(); i++)
+                if(plan_graphs_[i].adv == adv) {
+                    std::rotate(plan_graphs_.begin() + i, plan_graphs_.begin() + i + 1, plan_graphs_.end());
+                    x = plan_graphs_.back().x;
+                    break;
+                }
+            if(!x) {
+                if(plan_graphs_.size() >= MAX_PLAN_GRAPHS) {
+                    (void)hipGraphExecDestroy(plan_graphs_.front().x);
+                    (void)hipGraphDestroy(plan_graphs_.front().g);
+                    plan_graphs_.erase(plan_graphs_.begin());
+                }
+                PlanGraph p;
+                p.adv = adv;
+                e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+                if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
+                launch_all();
+                e = hipStreamEndCapture(stream_, &p.g);
+                if(e != hipSuccess) return hip_err("hipStreamEndCapture", e);
+                e = hipGraphInstantiate(&p.x, p.g, nullptr, nullptr, 0);
+                if(e != hipSuccess) { (void)hipGraphDestroy(p.g); return hip_err("hipGraphInstantiate", e); }
+                x = p.x;
+                plan_graphs_.push_back(std::move(p));
+            }
+        } else {
+            const int m = defer_ ? 1 : 0;
+            if(!exec_[m]) {
+                e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+                if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
+                launch_all();
+                e = hipStreamEndCapture(stream_, &graph_[m]);
+                if(e != hipSuccess) { graph_[m] = nullptr; return hip_err("hipStreamEndCapture", e); }
+                e = hipGraphInstantiate(&exec_[m], graph_[m], nullptr, nullptr, 0);
+                if(e != hipSuccess) { exec_[m] = nullptr; return hip_err("hipGraphInstantiate", e); }
+            }
+            x = exec_[m];
         }
-        e = hipGraphLaunch(exec_[m], stream_);
+        e = hipGraphLaunch(x, stream_);
         if(e != hipSuccess) return hip_err("hipGraphLaunch", e);
     } else {
         launch_all();
@@ -958,10 +998,22 @@ int64_t Batch::work_items(const Job &j) const
     return j.items;
 }
 
-void Batch::launch_finish(bool local)
+void Batch::launch_finish(bool local, int adv)
 {
     kern_finish(local ? n_lred_ : n_nred_, local ? d_lredw_ : d_nredw_, d_inst_, d_cfg_, d_state_,
-                local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, d_count_, local, stream_);
+                local ? d_lpart_ : d_npart_, local ? d_lred_ : d_nred_, d_count_, local, stream_, adv);
+}
+
+void Batch::plan_chunk(int64_t skip, int n, int replays, std::vector<int> &adv) const
+{
+    std::vector<InstCfg> cfg;
+    std::vector<InstState> st;
+    for(const Job *j : jobs_) {
+        if(!j || j->done) continue;
+        cfg.push_back(j->cfg);
+        st.push_back(j->state);
+    }
+    plan_finishes(cfg.data(), st.data(), (int)cfg.size(), skip, n, replays, adv);
 }
 
 void Batch::launch_step(bool local, const Defer *df)
@@ -995,13 +1047,14 @@ void Batch::launch_step(bool local, const Defer *df)
     }
 }
 
-void Batch::enqueue_steps(int n)
+void Batch::enqueue_steps(int n, const std::vector<int> *adv)
 {
     if(!defer_) {
         for(int t = 0; t < n; t++) {
             const bool local = (t & 1) == 0;     // every live job starts at an even step count
             launch_step(local);
-            launch_finish(local);
+            const int a = adv ? (*adv)[t] : 1;
+            if(a > 0) launch_finish(local, a);
         }
         return;
     }
@@ -1086,18 +1139,26 @@ std::string Batch::graph_steps(int steps, float &ms)
     hipGraph_t g[2] = {nullptr, nullptr};
     hipGraphExec_t x[2] = {nullptr, nullptr};
     hipEvent_t a = nullptr, b = nullptr;
-    auto capture = [&](int n, hipGraph_t &gg, hipGraphExec_t &xx) -> std::string {
+    // finish plans: the chunk graph runs 1 + reps times (one untimed), the
+    // remainder's once after them
+    const bool planned = !defer_ && sparse_ok_;
+    std::vector<int> adv[2];
+    if(planned) {
+        plan_chunk(0, c, 1 + reps, adv[0]);
+        if(rest) plan_chunk((int64_t)(1 + reps) * c, rest, 1, adv[1]);
+    }
+    auto capture = [&](int n, int k) -> std::string {
         hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
         if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
-        enqueue_steps(n);
-        e = hipStreamEndCapture(stream_, &gg);
-        if(e != hipSuccess) { gg = nullptr; return hip_err("hipStreamEndCapture", e); }
-        e = hipGraphInstantiate(&xx, gg, nullptr, nullptr, 0);
-        if(e != hipSuccess) { xx = nullptr; return hip_err("hipGraphInstantiate", e); }
+        enqueue_steps(n, planned ? &adv[k] : nullptr);
+        e = hipStreamEndCapture(stream_, &g[k]);
+        if(e != hipSuccess) { g[k] = nullptr; return hip_err("hipStreamEndCapture", e); }
+        e = hipGraphInstantiate(&x[k], g[k], nullptr, nullptr, 0);
+        if(e != hipSuccess) { x[k] = nullptr; return hip_err("hipGraphInstantiate", e); }
         return "";
     };
-    std::string err = capture(c, g[0], x[0]);
-    if(err.empty() && rest) err = capture(rest, g[1], x[1]);
+    std::string err = capture(c, 0);
+    if(err.empty() && rest) err = capture(rest, 1);
     if(err.empty() && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess))
         err = "hipEventCreate failed";
     if(err.empty()) {
@@ -1121,6 +1182,111 @@ std::string Batch::graph_steps(int steps, float &ms)
     if(e != hipSuccess) return hip_err("hipMemcpy(state)", e);
     for(int i = 0; i < n_inst_; i++) if(jobs_[i]) jobs_[i]->state = h_state_[i];
     return "";
+}
+
+// ---------------------------------------------------------------------------
+// finish plans
+// ---------------------------------------------------------------------------
+// the first step t > S.s after which a first phase may end: max_steps, or the
+// check (t % lag == 0) at which hold can first reach c.hold.  hold grows by at
+// most one per check, and only at checks at or after min_steps (a check before
+// them resets it).
+static int64_t first_possible_end(const InstCfg &c, const InstState &S)
+{
+    int64_t t = std::max(S.s + 1, c.max_steps);
+    const int64_t lim = (INT64_MAX / 4) / c.lag;
+    const int64_t b = S.s / c.lag;                   // checks so far: steps lag, ..., b * lag
+    int64_t k = -1;                                  // the check (index) that may end it
+    if(hold_can_grow(c)) {
+        const int64_t m = c.min_steps > 0 ? (c.min_steps - 1) / c.lag + 1 : 0;   // ceil(min_steps / lag)
+        k = std::max(b + 1, m) + std::max<int64_t>(1, (int64_t)c.hold - S.hold) - 1;
+    } else if(S.hold >= c.hold) {
+        k = b + 1;
+    }
+    if(k > 0 && k <= lim) t = std::min(t, k * c.lag);
+    return t;
+}
+
+void plan_finishes(const InstCfg *cfg, const InstState *st, int n_inst, int64_t skip, int n,
+                   int replays, std::vector<int> &adv)
+{
+    std::vector<uint8_t> need(n, 0);
+    need[n - 1] = 1;                                 // a chunk leaves every state current
+    for(int i = 0; i < n_inst; i++) {
+        const InstCfg &c = cfg[i];
+        const InstState &S = st[i];
+        if(S.phase >= PH_DONE) continue;
+        if(S.phase != PH_RUN) { std::fill(need.begin(), need.end(), 1); break; }
+        const int64_t tend = first_possible_end(c, S);
+        for(int r = 0; r < replays; r++) {
+            const int64_t t0 = S.s + skip + (int64_t)r * n;      // steps done before this run
+            for(int j = 0; j < n; j++) {
+                const int64_t t = t0 + j + 1;
+                if(t >= tend) { std::fill(need.begin() + std::max(j - 1, 0), need.end(), 1); break; }
+                if(!quiet_run_step(c, t)) { need[j] = 1; if(j > 0) need[j - 1] = 1; }
+            }
+        }
+    }
+    adv.assign(n, 0);
+    int run = 0;
+    for(int j = 0; j < n; j++) {
+        run++;
+        if(need[j]) { adv[j] = run; run = 0; }
+    }
+}
+
+int64_t verify_finish_plan(const std::vector<InstCfg> &cfg, const double *q, int64_t nsteps,
+                           const int64_t *start, int chunk, int replays, int plant,
+                           int64_t *finishes)
+{
+    const int n = (int)cfg.size();
+    std::vector<InstState> dense(n), sparse(n);
+    std::vector<uint8_t> joined(n, 0);
+    int64_t problems = 0, nfin = 0;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    // the flux instance i's step t (1-based) returns
+    auto flux_at = [&](int i, int64_t t) { return q + ((int64_t)i * nsteps + (t - 1)) * 3; };
+    const int64_t round = (int64_t)chunk * replays;
+    for(int64_t b = 0; b + round <= nsteps; b += round) {
+        std::vector<InstCfg> lc;
+        std::vector<InstState> ls;
+        for(int i = 0; i < n; i++) {
+            if(!joined[i] && start[i] <= b) {       // jobs join between chunks
+                init_state(cfg[i], dense[i]);
+                sparse[i] = dense[i];
+                joined[i] = 1;
+            }
+            if(joined[i] && sparse[i].phase < PH_DONE) { lc.push_back(cfg[i]); ls.push_back(sparse[i]); }
+        }
+        std::vector<int> adv;
+        plan_finishes(lc.data(), ls.data(), (int)lc.size(), 0, chunk, replays, adv);
+        if(plant) { adv.assign(chunk, 0); adv[chunk - 1] = chunk; }
+        for(int r = 0; r < replays; r++)
+            for(int j = 0; j < chunk; j++) {
+                nfin += adv[j] > 0;
+                for(int i = 0; i < n; i++) {
+                    if(!joined[i]) continue;
+                    InstState &D = dense[i], &P = sparse[i];
+                    if(D.phase >= PH_DONE) continue;             // the kernels skip it
+                    // what the step kernels read
+                    if(P.phase != D.phase || P.flux_next != D.flux_next || P.vel_next != D.vel_next)
+                        problems++;
+                    advance_state(cfg[i], D, D.flux_next ? flux_at(i, D.s + 1) : zero, D.flux_next);
+                    plan_next(cfg[i], D);
+                    if(adv[j] > 0 && P.phase < PH_DONE) {
+                        const int64_t t = P.s + (P.phase == PH_RUN ? adv[j] : 1);
+                        if(t > D.s) { problems++; continue; }
+                        advance_state_n(cfg[i], P, P.flux_next ? flux_at(i, t) : zero, P.flux_next, adv[j]);
+                        plan_next(cfg[i], P);
+                        if(std::memcmp(&P, &D, sizeof(D)) != 0) problems++;
+                    }
+                }
+            }
+        for(int i = 0; i < n; i++)
+            if(joined[i] && std::memcmp(&sparse[i], &dense[i], sizeof(InstState)) != 0) problems++;
+    }
+    if(finishes) *finishes = nfin;
+    return problems;
 }
 
 std::vector<Job *> Batch::take_done()

@@ -715,10 +715,10 @@ __device__ __forceinline__ void block_sum_i128(I128 (&s)[3], int &bad)
 // 128-bit sums, rounds q once, advances the state and resets the counter.
 // An instance with a single reduction item skips the count.
 // ---------------------------------------------------------------------------
-__device__ void monitor_update(const InstCfg &c, InstState *st, const double q[3], int flux)
+__device__ void monitor_update(const InstCfg &c, InstState *st, const double q[3], int flux, int adv)
 {
     InstState S = *st;
-    advance_state(c, S, q, flux);
+    advance_state_n(c, S, q, flux, adv);
     plan_next(c, S);
     *st = S;
 }
@@ -727,7 +727,7 @@ __global__ void __launch_bounds__(256)
 k_finish(const RedWork *__restrict__ work, const InstDesc *__restrict__ insts,
          const InstCfg *__restrict__ cfgs, InstState *__restrict__ st,
          const Partial *__restrict__ part, RPartial *__restrict__ red,
-         unsigned int *__restrict__ counter, int local)
+         unsigned int *__restrict__ counter, int local, int adv)
 {
     const RedWork w = work[blockIdx.x];
     __shared__ int sh_phase, sh_flux, sh_last;
@@ -790,7 +790,7 @@ k_finish(const RedWork *__restrict__ work, const InstDesc *__restrict__ insts,
             if(bad) q[0] = q[1] = q[2] = __builtin_nan("");
         }
         counter[w.inst] = 0u;                      // ready for the next step
-        monitor_update(c, &st[w.inst], q, flux);
+        monitor_update(c, &st[w.inst], q, flux, adv);
     }
 }
 
@@ -959,11 +959,11 @@ void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, cons
 
 void kern_finish(int nwork, const RedWork *work, const InstDesc *insts, const InstCfg *cfgs,
                  InstState *st, const Partial *part, RPartial *red, unsigned int *counter,
-                 bool local, hipStream_t s)
+                 bool local, hipStream_t s, int adv)
 {
     if(nwork <= 0) return;
     hipLaunchKernelGGL(k_finish, dim3(nwork), dim3(256), 0, s, work, insts, cfgs, st, part, red,
-                       counter, local ? 1 : 0);
+                       counter, local ? 1 : 0, adv);
 }
 
 } // namespace fp

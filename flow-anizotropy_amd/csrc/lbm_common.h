@@ -517,6 +517,34 @@ FP_HD void advance_state(const InstCfg &c, InstState &S, const double *q, int ha
     }
 }
 
+// Whether step t of a solve in its first phase (S.s becomes t) is quiet: the
+// step sums no flux and writes no velocity (plan_next), and the update after it
+// only counts it -- no block sample, no check, no end of the phase
+// (advance_state).  A run of quiet steps needs no finish of its own: the next
+// finish adds them (advance_state_n).
+FP_HD bool quiet_run_step(const InstCfg &c, int64_t t)
+{
+    if(t >= c.max_steps || t % c.lag == 0) return false;
+    if(c.kind == 2 && t % c.avg_sample == 0) return false;
+    return true;
+}
+
+// Whether a check can raise hold at all (3D: drift < eps |q|; 2D: d < tol with
+// tol from eps and ksigma): with a zero tolerance a first phase ends only at
+// max_steps, or when the hold it needs is already reached.
+FP_HD bool hold_can_grow(const InstCfg &c)
+{
+    return c.kind == 3 ? c.eps > 0.0 : (c.eps > 0.0 || c.ksigma > 0.0);
+}
+
+// advance by adv steps: adv - 1 quiet steps (quiet_run_step) and then the
+// step just taken, whose flux q is given when have_q
+FP_HD void advance_state_n(const InstCfg &c, InstState &S, const double *q, int have_q, int adv)
+{
+    if(adv > 1 && S.phase == PH_RUN) S.s += adv - 1;
+    advance_state(c, S, q, have_q);
+}
+
 // state of a fresh solve
 FP_HD void init_state(const InstCfg &c, InstState &S)
 {
@@ -544,6 +572,8 @@ constexpr int REDUCE_SPAN = 1024;
 // partials per step kernel (every workgroup of such an instance sums them on
 // the steps after a flux step)
 constexpr int DEFER_MAX_PARTIALS = 4096;
+// chunk graphs a batch keeps, one per finish plan (plan_finishes)
+constexpr size_t MAX_PLAN_GRAPHS = 8;
 struct RPartial {
     I128    s[3];
     int32_t bad;

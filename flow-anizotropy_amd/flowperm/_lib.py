@@ -183,6 +183,9 @@ SIGNATURES = {
                                          C.POINTER(Config3D), _dp, C.POINTER(SolveOut)]),
     "fp_verify_batch_tables": (C.c_int, [C.POINTER(_u8p), C.POINTER(C.c_int32), C.c_int, C.c_int,
                                          C.c_int, C.POINTER(Tuning), C.POINTER(C.c_int64)]),
+    "fp_verify_finish_plan": (C.c_int, [C.POINTER(Config3D), C.POINTER(Config2D), C.c_int, _dp,
+                                        C.c_int64, C.POINTER(C.c_int64), C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 }
 
 _lib = None

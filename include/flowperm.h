@@ -350,6 +350,23 @@ void fp_get_tuning(fp_ctx *ctx, fp_tuning *t);
 int fp_verify_batch_tables(const uint8_t *const *solids, const int32_t *dims, int n, int lattice,
                            int precision, const fp_tuning *tune, int64_t *problems);
 
+/* Self-check of the sparse finish schedule (no reference counterpart; host
+ * only, no device work).  A batch's chunks launch the per-step finish (flux
+ * sum and convergence update) only after steps on which some instance's rules
+ * can act, and the next finish counts the quiet steps in between.  This runs
+ * the convergence rules of n instances -- c3 (3D) or c2 (2D) configurations,
+ * exactly one non-NULL; q holds nsteps x 3 flux values per instance, step t
+ * (1-based) of instance i seeing q[(i*nsteps + t-1)*3 ..]; instance i joins
+ * at batch step start[i] -- once with a finish after every step and once with
+ * the finish schedules of chunks of `chunk` steps (even) replayed `replays`
+ * times, and counts in *problems every step whose outputs (phase, flux sum,
+ * velocity write) and every finish whose state differ.  *finishes: finishes
+ * of the schedules.  plant = 1 replaces each schedule by one finish at the
+ * chunk's end, a wrong schedule the check must catch. */
+int fp_verify_finish_plan(const fp_config3d *c3, const fp_config2d *c2, int n, const double *q,
+                          int64_t nsteps, const int64_t *start, int chunk, int replays, int plant,
+                          int64_t *problems, int64_t *finishes);
+
 /* Steps one resident instance: `warmup` then `steps` steps launched one by one
  * with events around every kernel (the per-kernel averages), then `steps` more
  * as hipGraph chunk replays (total_ms).  tune: NULL = the context's shapes.

@@ -1,0 +1,105 @@
+"""
+The sparse finish schedule on the device: a batch that launches the finish
+(exact flux sum and convergence update) after every step of its chunks gives
+bit for bit the results of one that launches it only after the steps its
+plan names (the default; FLOWPERM_SPARSE_FINISH=0 turns the plan off), for
+3D and 2D solves in every precision, with lags that do not divide the chunk,
+velocity fields, the convergence-check record, and a server batch whose
+requests join at different steps.  Both runs take the finish launches of the
+non-deferred path (FLOWPERM_DEFER_FINISH=0), where the plan applies.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_deferred import KEYS2, KEYS3
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(fn):
+    """fn() with a finish after every step, then with the finish plans"""
+    old = {k: os.environ.pop(k, None) for k in ("FLOWPERM_DEFER_FINISH", "FLOWPERM_SPARSE_FINISH")}
+    try:
+        os.environ["FLOWPERM_DEFER_FINISH"] = "0"
+        os.environ["FLOWPERM_SPARSE_FINISH"] = "0"
+        a = fn()
+        os.environ.pop("FLOWPERM_SPARSE_FINISH")
+        b = fn()
+    finally:
+        for k, v in old.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v
+    return a, b
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_sparse_finish_bitwise_3d(gpu_ctx, precision):
+    from flowperm import masks, solver
+    m = masks.spheres((24, 20, 28), 0.6, 2, 4, seed=5)
+    cfg = dict(ff=40, eps=1e-6, lag=37, min_steps=300, avg_steps=150, max_steps=3000,
+               precision=precision)
+    print(f"3d {precision}", flush=True)
+    a, b = _both(lambda: solver.solve_permeability3d(m, ctx=gpu_ctx, velocity=True, **cfg))
+    for k in KEYS3:
+        assert a[k] == b[k], k
+    assert np.array_equal(a["velocity"], b["velocity"])
+
+
+def test_sparse_finish_bitwise_max_steps_3d(gpu_ctx):
+    """a first phase that ends at max_steps (not a multiple of the lag) in
+    the middle of a chunk"""
+    from flowperm import masks, solver
+    m = masks.ellipsoids((32, 32, 32), 0.6, seed=2)
+    cfg = dict(ff=40, eps=0.0, lag=150, min_steps=0, avg_steps=90, max_steps=733, precision="f32")
+    print("3d max_steps", flush=True)
+    a, b = _both(lambda: solver.solve_permeability3d(m, ctx=gpu_ctx, velocity=True, **cfg))
+    for k in KEYS3:
+        assert a[k] == b[k], k
+    assert a["steps_fx"] == 733 + 90
+    assert np.array_equal(a["velocity"], b["velocity"])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32ref"])
+def test_sparse_finish_bitwise_2d(gpu_ctx, precision):
+    from flowperm import masks, solver
+    m = masks.ellipses((48, 64), 0.6, rmin=1.5, rmax=3.0, seed=12, axis=(1, 2))
+    cfg = dict(lag=130, min_steps=400, max_steps=3000, avg_steps=300, avg_sample=7, validate=30.0)
+    print(f"2d {precision}", flush=True)
+    a, b = _both(lambda: solver.solve_permeability2d(m, ctx=gpu_ctx, precision=precision,
+                                                     velocity=True, **cfg))
+    for k in KEYS2 + ["val_q", "val_steps"]:
+        assert a[k] == b[k], k
+    assert np.array_equal(a["velocity"], b["velocity"])
+
+
+def test_sparse_finish_bitwise_staggered_server():
+    """requests that join a running batch at different steps, with different
+    lags: each chunk's plan is the union of the instances' events"""
+    from flowperm import masks, solver
+    from flowperm.server import Server
+    ms = [masks.spheres((20, 20, 24), 0.7, 2, 4, seed=s) for s in (1, 2, 3)]
+    cfgs = [dict(ff=40, eps=1e-6, lag=lag, min_steps=200, avg_steps=120, max_steps=2500)
+            for lag in (50, 77, 300)]
+
+    def run():
+        ctx = solver.Context()
+        try:
+            with Server(3, "f32", ctx=ctx) as srv:
+                ids = [srv.submit(ms[0], **cfgs[0])]
+                for m, c in zip(ms[1:], cfgs[1:]):
+                    for _ in range(2):
+                        srv.step()
+                    ids.append(srv.submit(m, **c))
+                while srv.step():
+                    pass
+                return [srv.poll(i)[0] for i in ids]
+        finally:
+            ctx.close()
+    print("staggered server", flush=True)
+    a, b = _both(run)
+    for ra, rb in zip(a, b):
+        for k in KEYS3:
+            assert ra[k] == rb[k], k
