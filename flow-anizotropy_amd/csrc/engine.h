@@ -206,8 +206,8 @@ private:
     Tuning tune_;
     hipGraph_t graph_[2] = {nullptr, nullptr};       // chunk graphs: finish per step, deferred
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
-    // chunk graphs of the finish plans met since the last rebuild (the plan
-    // of a stable job set repeats with the lag; most recently used last)
+    // chunk graphs of the first finish plans met since the last rebuild (the
+    // plans of a stable job set repeat with the lags)
     struct PlanGraph {
         std::vector<int> adv;
         hipGraph_t g = nullptr;

@@ -919,55 +919,56 @@ std::string Batch::run_chunk()
     const double t0 = now_seconds();
     hipError_t e;
     defer_ = choose_defer();
+    // the finish plan of this chunk (null: a finish after every step)
     std::vector<int> adv;
-    const bool planned = !defer_ && sparse_ok_;
-    if(planned) plan_chunk(0, chunk_, 1, adv);
-    auto launch_all = [&]() { enqueue_steps(chunk_, planned ? &adv : nullptr); };
+    const std::vector<int> *plan = nullptr;
+    if(!defer_ && sparse_ok_) {
+        plan_chunk(0, chunk_, 1, adv);
+        if(std::any_of(adv.begin(), adv.end(), [](int a) { return a != 1; })) plan = &adv;
+    }
+    auto capture = [&](const std::vector<int> *pl, hipGraph_t &g, hipGraphExec_t &x) -> std::string {
+        hipError_t ce = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+        if(ce != hipSuccess) return hip_err("hipStreamBeginCapture", ce);
+        enqueue_steps(chunk_, pl);
+        ce = hipStreamEndCapture(stream_, &g);
+        if(ce != hipSuccess) { g = nullptr; return hip_err("hipStreamEndCapture", ce); }
+        ce = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if(ce != hipSuccess) { x = nullptr; return hip_err("hipGraphInstantiate", ce); }
+        return "";
+    };
     if(use_graph_) {
         hipGraphExec_t x = nullptr;
-        if(planned) {
-            // the graph of this plan: cached, most recently used last
-            for(size_t i = 0; i < plan_graphs_.size(); i++)
-                if(plan_graphs_[i].adv == adv) {
-                    std::rotate(plan_graphs_.begin() + i, plan_graphs_.begin() + i + 1, plan_graphs_.end());
-                    x = plan_graphs_.back().x;
-                    break;
+        if(plan) {
+            for(const PlanGraph &pg : plan_graphs_)
+                if(pg.adv == adv) { x = pg.x; break; }
+            // A job set's plans repeat with the lags, so a few graphs cover
+            // them; lags that do not divide the chunk can give more plans than
+            // that, and a full cache then runs the chunk with a finish after
+            // every step rather than capturing a graph per chunk.
+            if(!x && plan_graphs_.size() < MAX_PLAN_GRAPHS) {
+                PlanGraph pg;
+                pg.adv = adv;
+                std::string err = capture(plan, pg.g, pg.x);
+                if(!err.empty()) {
+                    if(pg.g) (void)hipGraphDestroy(pg.g);
+                    return err;
                 }
-            if(!x) {
-                if(plan_graphs_.size() >= MAX_PLAN_GRAPHS) {
-                    (void)hipGraphExecDestroy(plan_graphs_.front().x);
-                    (void)hipGraphDestroy(plan_graphs_.front().g);
-                    plan_graphs_.erase(plan_graphs_.begin());
-                }
-                PlanGraph p;
-                p.adv = adv;
-                e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
-                if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
-                launch_all();
-                e = hipStreamEndCapture(stream_, &p.g);
-                if(e != hipSuccess) return hip_err("hipStreamEndCapture", e);
-                e = hipGraphInstantiate(&p.x, p.g, nullptr, nullptr, 0);
-                if(e != hipSuccess) { (void)hipGraphDestroy(p.g); return hip_err("hipGraphInstantiate", e); }
-                x = p.x;
-                plan_graphs_.push_back(std::move(p));
+                x = pg.x;
+                plan_graphs_.push_back(std::move(pg));
             }
-        } else {
+        }
+        if(!x) {
             const int m = defer_ ? 1 : 0;
             if(!exec_[m]) {
-                e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
-                if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
-                launch_all();
-                e = hipStreamEndCapture(stream_, &graph_[m]);
-                if(e != hipSuccess) { graph_[m] = nullptr; return hip_err("hipStreamEndCapture", e); }
-                e = hipGraphInstantiate(&exec_[m], graph_[m], nullptr, nullptr, 0);
-                if(e != hipSuccess) { exec_[m] = nullptr; return hip_err("hipGraphInstantiate", e); }
+                std::string err = capture(nullptr, graph_[m], exec_[m]);
+                if(!err.empty()) return err;
             }
             x = exec_[m];
         }
         e = hipGraphLaunch(x, stream_);
         if(e != hipSuccess) return hip_err("hipGraphLaunch", e);
     } else {
-        launch_all();
+        enqueue_steps(chunk_, plan);
         e = hipGetLastError();
         if(e != hipSuccess) return hip_err("kernel launch", e);
     }
