@@ -13,6 +13,7 @@
 // stream; the results are bit-identical to fp_solve_dir3d's.
 #include "solve_common.h"
 
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -174,22 +175,30 @@ extern "C" int fp_solve_dir3d_noalloc(fp_ctx *ctx, const uint8_t *solid, int nx,
     const int n_l = (int)t.lw.size(), n_n = (int)t.nw.size();
     const int n_lr = (int)t.lr.size(), n_nr = (int)t.nr.size();
     const int chunk = std::max(2, ctx->chunk & ~1);
+    // a finish only after the steps on which the convergence rules can act
+    // (plan_finishes), as in a batch; FLOWPERM_SPARSE_FINISH=0: after every step
+    const char *sv = getenv("FLOWPERM_SPARSE_FINISH");
+    const bool sparse = !(sv && sv[0] == '0');
+    std::vector<int> adv(chunk, 1);
     while(S.phase < PH_DONE) {
+        if(sparse) plan_finishes(&j.cfg, &S, 1, 0, chunk, 1, adv);
         // every chunk starts at an even step count: local (odd) step first
         for(int step = 0; step < chunk; step++) {
             const bool local = (step & 1) == 0;
             if(local) {
                 kern_step(3, prec, true, tu, AD_B32, d_inst, d_state, base + L.lwork, n_l,
                           (Partial *)(base + L.lpart), s);
-                kern_finish(n_lr, (const RedWork *)(base + L.lredw), d_inst, d_cfg, d_state,
-                            (const Partial *)(base + L.lpart), (RPartial *)(base + L.lred),
-                            (unsigned int *)(base + L.count), true, s);
+                if(adv[step] > 0)
+                    kern_finish(n_lr, (const RedWork *)(base + L.lredw), d_inst, d_cfg, d_state,
+                                (const Partial *)(base + L.lpart), (RPartial *)(base + L.lred),
+                                (unsigned int *)(base + L.count), true, s, adv[step]);
             } else {
                 kern_step(3, prec, false, tu, AD_B32, d_inst, d_state, base + L.nwork, n_n,
                           (Partial *)(base + L.npart), s);
-                kern_finish(n_nr, (const RedWork *)(base + L.nredw), d_inst, d_cfg, d_state,
-                            (const Partial *)(base + L.npart), (RPartial *)(base + L.nred),
-                            (unsigned int *)(base + L.count), false, s);
+                if(adv[step] > 0)
+                    kern_finish(n_nr, (const RedWork *)(base + L.nredw), d_inst, d_cfg, d_state,
+                                (const Partial *)(base + L.npart), (RPartial *)(base + L.nred),
+                                (unsigned int *)(base + L.count), false, s, adv[step]);
             }
         }
         he = hipGetLastError();
