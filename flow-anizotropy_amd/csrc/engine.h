@@ -165,6 +165,11 @@ private:
     // the finish plan of the next n steps, run `replays` times back to back
     // after `skip` steps, from the jobs' current states
     void plan_chunk(int64_t skip, int n, int replays, std::vector<int> &adv) const;
+    // whether those steps run with that plan (true), or else with the deferred
+    // finish (can_defer) or a finish after every step: a plan unless it has a
+    // finish after every step, or the deferred finish applies and the plan has
+    // more than one finish per PLAN_OVER_DEFER steps
+    bool choose_plan(int64_t skip, int n, int replays, std::vector<int> &adv, bool &can_defer) const;
     bool wants_rebuild() const;
     bool choose_defer() const;
     int64_t work_items(const Job &j) const;
@@ -182,7 +187,7 @@ private:
     InstCfg *d_cfg_ = nullptr;
     InstState *d_state_ = nullptr;
     InstState *d_state2_ = nullptr;          // the other state array of the deferred finish
-    bool defer_ = false;                     // the mode of the last chunk
+    bool defer_ = false;                     // the mode of the last chunk (or of enqueue_steps)
     bool defer_ok_ = false;                  // every instance within DEFER_MAX_PARTIALS
     bool defer_small_ = false;               // ... and within REDUCE_SPAN
     LocalWork *d_lwork_ = nullptr;

@@ -918,14 +918,11 @@ std::string Batch::run_chunk()
     if(n_inst_ == 0) return "";
     const double t0 = now_seconds();
     hipError_t e;
-    defer_ = choose_defer();
-    // the finish plan of this chunk (null: a finish after every step)
+    // the chunk's finish plan, or (null) the deferred finish where it
+    // applies, else a finish after every step
     std::vector<int> adv;
-    const std::vector<int> *plan = nullptr;
-    if(!defer_ && sparse_ok_) {
-        plan_chunk(0, chunk_, 1, adv);
-        if(std::any_of(adv.begin(), adv.end(), [](int a) { return a != 1; })) plan = &adv;
-    }
+    bool can_defer = false;
+    const std::vector<int> *plan = choose_plan(0, chunk_, 1, adv, can_defer) ? &adv : nullptr;
     auto capture = [&](const std::vector<int> *pl, hipGraph_t &g, hipGraphExec_t &x) -> std::string {
         hipError_t ce = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
         if(ce != hipSuccess) return hip_err("hipStreamBeginCapture", ce);
@@ -938,13 +935,14 @@ std::string Batch::run_chunk()
     };
     if(use_graph_) {
         hipGraphExec_t x = nullptr;
+        defer_ = false;
         if(plan) {
             for(const PlanGraph &pg : plan_graphs_)
                 if(pg.adv == adv) { x = pg.x; break; }
             // A job set's plans repeat with the lags, so a few graphs cover
             // them; lags that do not divide the chunk can give more plans than
-            // that, and a full cache then runs the chunk with a finish after
-            // every step rather than capturing a graph per chunk.
+            // that, and a full cache then runs the chunk without a plan rather
+            // than capturing a graph per chunk.
             if(!x && plan_graphs_.size() < MAX_PLAN_GRAPHS) {
                 PlanGraph pg;
                 pg.adv = adv;
@@ -958,6 +956,7 @@ std::string Batch::run_chunk()
             }
         }
         if(!x) {
+            defer_ = can_defer;
             const int m = defer_ ? 1 : 0;
             if(!exec_[m]) {
                 std::string err = capture(nullptr, graph_[m], exec_[m]);
@@ -968,6 +967,7 @@ std::string Batch::run_chunk()
         e = hipGraphLaunch(x, stream_);
         if(e != hipSuccess) return hip_err("hipGraphLaunch", e);
     } else {
+        defer_ = !plan && can_defer;
         enqueue_steps(chunk_, plan);
         e = hipGetLastError();
         if(e != hipSuccess) return hip_err("kernel launch", e);
@@ -1015,6 +1015,18 @@ void Batch::plan_chunk(int64_t skip, int n, int replays, std::vector<int> &adv) 
         st.push_back(j->state);
     }
     plan_finishes(cfg.data(), st.data(), (int)cfg.size(), skip, n, replays, adv);
+}
+
+bool Batch::choose_plan(int64_t skip, int n, int replays, std::vector<int> &adv, bool &can_defer) const
+{
+    can_defer = choose_defer();
+    if(!sparse_ok_) return false;
+    plan_chunk(skip, n, replays, adv);
+    const int64_t nfin = std::count_if(adv.begin(), adv.end(), [](int a) { return a > 0; });
+    if(nfin >= n) return false;                      // a finish after every step: no plan
+    // the deferred update costs every step kernel 0.7-2 us on the lattices it
+    // takes, a finish launch 4-8 us: a plan wins while it keeps few finishes
+    return !(can_defer && nfin * PLAN_OVER_DEFER > n);
 }
 
 void Batch::launch_step(bool local, const Defer *df)
@@ -1134,24 +1146,28 @@ std::string Batch::graph_steps(int steps, float &ms)
     if(n_inst_ == 0) return "nothing to step";
     steps &= ~1;
     if(steps < 2) return "";
-    defer_ = choose_defer();
     const int c = std::min(steps, 200) & ~1;         // every replay starts at an even step
     const int reps = steps / c, rest = steps - reps * c;
     hipGraph_t g[2] = {nullptr, nullptr};
     hipGraphExec_t x[2] = {nullptr, nullptr};
     hipEvent_t a = nullptr, b = nullptr;
-    // finish plans: the chunk graph runs 1 + reps times (one untimed), the
-    // remainder's once after them
-    const bool planned = !defer_ && sparse_ok_;
+    // the modes of the two graphs: the chunk graph runs 1 + reps times (one
+    // untimed), the remainder's once after them
     std::vector<int> adv[2];
-    if(planned) {
-        plan_chunk(0, c, 1 + reps, adv[0]);
-        if(rest) plan_chunk((int64_t)(1 + reps) * c, rest, 1, adv[1]);
+    const std::vector<int> *plan[2] = {nullptr, nullptr};
+    bool dfr[2] = {false, false};
+    for(int k = 0; k < (rest ? 2 : 1); k++) {
+        bool cd = false;
+        if(choose_plan(k ? (int64_t)(1 + reps) * c : 0, k ? rest : c, k ? 1 : 1 + reps, adv[k], cd))
+            plan[k] = &adv[k];
+        else
+            dfr[k] = cd;
     }
     auto capture = [&](int n, int k) -> std::string {
+        defer_ = dfr[k];
         hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
         if(e != hipSuccess) return hip_err("hipStreamBeginCapture", e);
-        enqueue_steps(n, planned ? &adv[k] : nullptr);
+        enqueue_steps(n, plan[k]);
         e = hipStreamEndCapture(stream_, &g[k]);
         if(e != hipSuccess) { g[k] = nullptr; return hip_err("hipStreamEndCapture", e); }
         e = hipGraphInstantiate(&x[k], g[k], nullptr, nullptr, 0);
@@ -1160,6 +1176,7 @@ std::string Batch::graph_steps(int steps, float &ms)
     };
     std::string err = capture(c, 0);
     if(err.empty() && rest) err = capture(rest, 1);
+    defer_ = dfr[0];                                 // the mode of the timed replays
     if(err.empty() && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess))
         err = "hipEventCreate failed";
     if(err.empty()) {

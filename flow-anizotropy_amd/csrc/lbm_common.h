@@ -574,6 +574,9 @@ constexpr int REDUCE_SPAN = 1024;
 constexpr int DEFER_MAX_PARTIALS = 4096;
 // chunk graphs a batch keeps, one per finish plan (plan_finishes)
 constexpr size_t MAX_PLAN_GRAPHS = 8;
+// where the deferred finish applies, a chunk takes a finish plan only if it
+// has at most one finish per this many steps
+constexpr int PLAN_OVER_DEFER = 5;
 struct RPartial {
     I128    s[3];
     int32_t bad;

@@ -3,11 +3,12 @@ The deferred finish: on lattices with at most 1024 work items per step kernel
 (up to 4096 while the flux is summed on few steps) each step kernel applies
 the previous step's flux reduction and convergence update itself (every
 workgroup for its own instance), and a chunk ends with a single finish launch,
-instead of a finish launch after every step.  It must
-not change a single bit: every result equals the one with a finish launch
-after every step (FLOWPERM_DEFER_FINISH=0), for both neighbour-step forms, a
-batch mixing them, 3D and 2D in every precision, with the velocity fields and
-the convergence-check record.  fp_bench_steps reports which path ran.
+instead of finish launches.  It must not change a single bit: every result
+with the deferred finish on every chunk it applies to (finish plans off,
+FLOWPERM_SPARSE_FINISH=0) equals the one with finish launches
+(FLOWPERM_DEFER_FINISH=0), for both neighbour-step forms, a batch mixing them,
+3D and 2D in every precision, with the velocity fields and the
+convergence-check record.  fp_bench_steps reports which path ran.
 """
 import ctypes as C
 import os
@@ -24,18 +25,27 @@ KEYS2 = ["k_xx", "k_xy", "k_yx", "k_yy", "steps_fx", "steps_fy", "fieldres_fx", 
          "check_fx", "check_fy"] + [f"{p}q{a}_{t}" for p in ("", "se_") for a in "xy" for t in ("fx", "fy")]
 
 
-def _both(fn):
-    """fn() with the deferred finish, then with a finish launch per step"""
-    old = os.environ.pop("FLOWPERM_DEFER_FINISH", None)
+ENV = ("FLOWPERM_DEFER_FINISH", "FLOWPERM_SPARSE_FINISH")
+
+
+def _with(env, fn):
+    """fn() with exactly these finish settings"""
+    old = {k: os.environ.pop(k, None) for k in ENV}
     try:
-        a = fn()
-        os.environ["FLOWPERM_DEFER_FINISH"] = "0"
-        b = fn()
+        os.environ.update(env)
+        return fn()
     finally:
-        os.environ.pop("FLOWPERM_DEFER_FINISH", None)
-        if old is not None:
-            os.environ["FLOWPERM_DEFER_FINISH"] = old
-    return a, b
+        for k, v in old.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v
+
+
+def _both(fn):
+    """fn() with the deferred finish wherever it applies, then with finish
+    launches"""
+    return (_with({"FLOWPERM_SPARSE_FINISH": "0"}, fn),
+            _with({"FLOWPERM_DEFER_FINISH": "0"}, fn))
 
 
 def _ctx(mode):
@@ -48,7 +58,7 @@ def _ctx(mode):
     return ctx
 
 
-def _bench(ctx, mask, lattice, prec):
+def _bench(ctx, mask, lattice, prec, flux_every_step=0):
     from flowperm import _lib
     if lattice == 3:
         nz, ny, nx = mask.shape
@@ -58,24 +68,32 @@ def _bench(ctx, mask, lattice, prec):
     F = np.array([1e-5, 0.0, 0.0])
     m = np.ascontiguousarray(mask, np.uint8)
     st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(m), nx, ny, nz, lattice, prec, _lib.f64ptr(F),
-                                2, 20, None, 0, C.byref(out))
+                                2, 20, None, flux_every_step, C.byref(out))
     assert st == 0, ctx.last_error()
     return out
 
 
 def test_the_deferred_path_runs_where_it_applies(gpu_ctx):
-    """up to 1024 partials per step kernel: always; up to 4096: while the
-    flux is summed on few steps (the bench's steps: a check every 2000);
-    more: never"""
+    """Without finish plans: up to 1024 partials per step kernel always; up
+    to 4096 while the flux is summed on few steps (the bench's steps: a check
+    every 2000); more never.  With them (the default), quiet steps take the
+    plan, whose few finish launches cost less than the deferred update in
+    every step kernel, and steps that sum the flux every time take the
+    deferred finish where it applies (up to 1024 partials)."""
     from flowperm import masks
     small3 = masks.spheres((24, 20, 28), 0.6, 2, 4, seed=3)
     small2 = masks.ellipses((256, 256), 0.6, seed=3)
     medium3 = masks.spheres((80, 80, 80), 0.6, 3, 6, seed=3)     # ~600 local, 3200 neighbour items
     large3 = masks.spheres((160, 160, 160), 0.6, 3, 6, seed=3)   # ~4800 local items
+    cases = ((small3, 3), (small2, 2), (medium3, 3), (large3, 3))
     print("deferred flags", flush=True)
-    a, b = _both(lambda: tuple(_bench(gpu_ctx, m, lat, 1).deferred
-                               for m, lat in ((small3, 3), (small2, 2), (medium3, 3), (large3, 3))))
+
+    def flags(flux):
+        return tuple(_bench(gpu_ctx, m, lat, 1, flux).deferred for m, lat in cases)
+    a, b = _both(lambda: flags(0))
     assert a == (1, 1, 1, 0) and b == (0, 0, 0, 0)
+    assert _with({}, lambda: flags(0)) == (0, 0, 0, 0)
+    assert _with({}, lambda: flags(1)) == (1, 1, 0, 0)
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2], ids=["row_segments", "compact", "per_structure"])

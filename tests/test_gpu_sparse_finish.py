@@ -75,6 +75,44 @@ def test_sparse_finish_bitwise_2d(gpu_ctx, precision):
     assert np.array_equal(a["velocity"], b["velocity"])
 
 
+@pytest.mark.parametrize("lattice", [3, 2])
+def test_default_modes_bitwise_small_lattices(gpu_ctx, lattice):
+    """a small lattice with the default settings: its first-phase chunks
+    take finish plans, its averaging chunks (3D: the flux on every step) the
+    deferred finish, and the solve equals the one with a finish after every
+    step"""
+    from flowperm import masks, solver
+    if lattice == 3:
+        m = masks.spheres((24, 20, 28), 0.6, 2, 4, seed=7)
+        cfg = dict(ff=40, eps=1e-6, lag=100, min_steps=300, avg_steps=250, max_steps=3000)
+
+        def run():
+            return solver.solve_permeability3d(m, ctx=gpu_ctx, velocity=True, precision="f32", **cfg)
+        keys = KEYS3
+    else:
+        m = masks.ellipses((48, 64), 0.6, rmin=1.5, rmax=3.0, seed=13, axis=(1, 2))
+        cfg = dict(lag=130, min_steps=400, max_steps=3000, avg_steps=300, avg_sample=7)
+
+        def run():
+            return solver.solve_permeability2d(m, ctx=gpu_ctx, precision="f32", velocity=True, **cfg)
+        keys = KEYS2
+    print(f"default modes {lattice}d", flush=True)
+    a = run()
+    old = {k: os.environ.pop(k, None) for k in ("FLOWPERM_DEFER_FINISH", "FLOWPERM_SPARSE_FINISH")}
+    try:
+        os.environ["FLOWPERM_DEFER_FINISH"] = "0"
+        os.environ["FLOWPERM_SPARSE_FINISH"] = "0"
+        b = run()
+    finally:
+        for k, v in old.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v
+    for k in keys:
+        assert a[k] == b[k], k
+    assert np.array_equal(a["velocity"], b["velocity"])
+
+
 def test_sparse_finish_bitwise_staggered_server():
     """requests that join a running batch at different steps, with different
     lags: each chunk's plan is the union of the instances' events"""
