@@ -245,11 +245,12 @@ std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,
 // after step t advances the state by adv[t] steps -- the quiet steps since the
 // previous finish and step t (advance_state_n); adv[t] = 0: no finish.  A
 // finish follows every step that is not quiet for some live instance
-// (quiet_run_step) and every step before one (its finish plans that step's
-// outputs), the last step of the chunk, and every step from the first at
-// which an instance's first phase may end (max_steps, or the check at which
-// hold can first reach c.hold).  Instances in their averaging phase take a
-// finish after every step; retired ones (phase >= PH_DONE) none.
+// (quiet_run_step, quiet_avg_step) and every step before one (its finish
+// plans that step's outputs), the last step of the chunk, and every step from
+// the first at which an instance's first phase may end (max_steps, or the
+// check at which hold can first reach c.hold).  3D instances in their
+// averaging phase (a flux sum every step) take a finish after every step;
+// retired ones (phase >= PH_DONE) none.
 void plan_finishes(const InstCfg *cfg, const InstState *st, int n_inst, int64_t skip, int n,
                    int replays, std::vector<int> &adv);
 // host only: a batch of n instances with the flux sequences q (nsteps x 3 per

@@ -1234,7 +1234,20 @@ void plan_finishes(const InstCfg *cfg, const InstState *st, int n_inst, int64_t 
         const InstCfg &c = cfg[i];
         const InstState &S = st[i];
         if(S.phase >= PH_DONE) continue;
-        if(S.phase != PH_RUN) { std::fill(need.begin(), need.end(), 1); break; }
+        if(S.phase == PH_AVG) {
+            if(c.kind == 3) { std::fill(need.begin(), need.end(), 1); break; }
+            // 2D: a sample every avg_sample steps, the end at avg_steps
+            for(int r = 0; r < replays; r++) {
+                const int64_t a0 = S.tavg + skip + (int64_t)r * n;   // averaging steps done before this run
+                if(a0 >= c.avg_steps) break;
+                for(int j = 0; j < n; j++) {
+                    const int64_t ta = a0 + j + 1;
+                    if(!quiet_avg_step(c, ta)) { need[j] = 1; if(j > 0) need[j - 1] = 1; }
+                    if(ta >= c.avg_steps) break;                     // done after this step
+                }
+            }
+            continue;
+        }
         const int64_t tend = first_possible_end(c, S);
         for(int r = 0; r < replays; r++) {
             const int64_t t0 = S.s + skip + (int64_t)r * n;      // steps done before this run

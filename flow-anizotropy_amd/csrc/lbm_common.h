@@ -537,11 +537,26 @@ FP_HD bool hold_can_grow(const InstCfg &c)
     return c.kind == 3 ? c.eps > 0.0 : (c.eps > 0.0 || c.ksigma > 0.0);
 }
 
-// advance by adv steps: adv - 1 quiet steps (quiet_run_step) and then the
-// step just taken, whose flux q is given when have_q
+// Whether step ta of a 2D averaging phase (S.tavg becomes ta) is quiet: no
+// flux sum, no velocity write, no sample and not the last step.  A 3D
+// averaging phase sums the flux on every step and has no quiet step.
+FP_HD bool quiet_avg_step(const InstCfg &c, int64_t ta)
+{
+    return c.kind == 2 && ta % c.avg_sample != 0 && ta < c.avg_steps;
+}
+
+// advance by adv steps: adv - 1 quiet steps (quiet_run_step, quiet_avg_step)
+// and then the step just taken, whose flux q is given when have_q
 FP_HD void advance_state_n(const InstCfg &c, InstState &S, const double *q, int have_q, int adv)
 {
-    if(adv > 1 && S.phase == PH_RUN) S.s += adv - 1;
+    if(adv > 1) {
+        if(S.phase == PH_RUN) {
+            S.s += adv - 1;
+        } else if(S.phase == PH_AVG && c.kind == 2) {
+            S.s += adv - 1;
+            S.tavg += adv - 1;
+        }
+    }
     advance_state(c, S, q, have_q);
 }
 

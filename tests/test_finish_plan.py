@@ -142,3 +142,25 @@ def test_quiet_chunks_take_one_or_two_finishes():
     prob, fin = run(cfg, 3, q, [0], 300, 1)
     assert prob == 0
     assert fin == 10 + 3 + 3 + 2 + (300 - 98) + 2
+
+
+def test_2d_averaging_phase_takes_finishes_around_its_samples():
+    """a 2D averaging phase (LMB2d/main.cpp:194-206) sums the flux every
+    avg_sample steps: its chunks need finishes at its samples only.  max_steps
+    200 ends the first phase at the end of the first chunk; 1000 averaging
+    steps with a sample every 50 follow.  Chunks of 200: the first phase's
+    chunk has its 4 block samples (and the end at 200), each averaging chunk
+    its 4 samples (the last at the phase's end), each a finish before and
+    after: 8 finishes; the two chunks after the end keep their closing one."""
+    from flowperm import _lib
+    cfg = (_lib.Config2D * 1)()
+    _lib.load().fp_config2d_default(cfg)
+    c = cfg[0]
+    c.eps, c.ksigma, c.L0 = 0.0, 0.0, 1.0
+    c.lag, c.min_steps, c.max_steps = 1000, 0, 200
+    c.avg_steps, c.avg_sample = 1000, 50
+    q = np.ones((1, 1600, 3))
+    q[:, :, 2] = 0.0
+    prob, fin = run(cfg, 2, q, [0], 200, 1)
+    assert prob == 0
+    assert fin == 6 * 8 + 2
