@@ -1208,18 +1208,18 @@ std::string Batch::graph_steps(int steps, float &ms)
 // the first step t > S.s after which a first phase may end: max_steps, or the
 // check (t % lag == 0) at which hold can first reach c.hold.  hold grows by at
 // most one per check, and only at checks at or after min_steps (a check before
-// them resets it).
+// them resets it); with c.hold <= 0 any check ends the phase.
 static int64_t first_possible_end(const InstCfg &c, const InstState &S)
 {
     int64_t t = std::max(S.s + 1, c.max_steps);
     const int64_t lim = (INT64_MAX / 4) / c.lag;
     const int64_t b = S.s / c.lag;                   // checks so far: steps lag, ..., b * lag
     int64_t k = -1;                                  // the check (index) that may end it
-    if(hold_can_grow(c)) {
-        const int64_t m = c.min_steps > 0 ? (c.min_steps - 1) / c.lag + 1 : 0;   // ceil(min_steps / lag)
-        k = std::max(b + 1, m) + std::max<int64_t>(1, (int64_t)c.hold - S.hold) - 1;
-    } else if(S.hold >= c.hold) {
+    if(c.hold <= 0 || S.hold >= c.hold) {
         k = b + 1;
+    } else if(hold_can_grow(c)) {
+        const int64_t m = c.min_steps > 0 ? (c.min_steps - 1) / c.lag + 1 : 0;   // ceil(min_steps / lag)
+        k = std::max(b + 1, m) + ((int64_t)c.hold - S.hold) - 1;
     }
     if(k > 0 && k <= lim) t = std::min(t, k * c.lag);
     return t;

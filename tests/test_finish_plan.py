@@ -41,7 +41,7 @@ def configs3d(rng, n):
     for c in out:
         c.eps = float(rng.choice([0.0, 1e-4, 1e-3, 1e-2, 0.05]))
         c.lag = int(rng.choice([1, 2, 3, 7, 16, 50, 199]))
-        c.hold = int(rng.choice([1, 2, 3]))
+        c.hold = int(rng.choice([-1, 0, 1, 1, 2, 3]))
         c.min_steps = int(rng.choice([0, 20, 300]))
         c.max_steps = int(rng.choice([150, 701, 5000]))
         c.avg_steps = int(rng.choice([0, 1, 5, 40]))
@@ -58,7 +58,7 @@ def configs2d(rng, n):
         c.L0 = 1.0
         c.lag = int(rng.choice([1, 4, 9, 30, 120]))
         c.avg_sample = int(rng.choice([1, 3, 10]))
-        c.hold = int(rng.choice([1, 2, 3]))
+        c.hold = int(rng.choice([-1, 0, 1, 1, 2, 3]))
         c.min_steps = int(rng.choice([0, 50, 400]))
         c.max_steps = int(rng.choice([160, 903, 5000]))
         c.avg_steps = int(rng.choice([0, 7, 40, 150]))
@@ -164,3 +164,40 @@ def test_2d_averaging_phase_takes_finishes_around_its_samples():
     prob, fin = run(cfg, 2, q, [0], 200, 1)
     assert prob == 0
     assert fin == 6 * 8 + 2
+
+
+def test_hold_zero_ends_the_first_phase_at_any_check():
+    """with --hold 0 the first check ends the first phase whatever min_steps
+    says (hold >= 0 holds after the reset, LMB3d/lbm3d-perm.cpp:345-352): lag
+    50, min_steps 1000.  The first chunk is dense from the step before the
+    check at 50 (152 finishes); the 100 averaging steps end at 150; the two
+    chunks after keep their closing finish."""
+    from flowperm import _lib
+    cfg = (_lib.Config3D * 1)()
+    _lib.load().fp_config3d_default(cfg)
+    c = cfg[0]
+    c.eps, c.hold, c.lag, c.min_steps, c.max_steps, c.avg_steps = 1e-3, 0, 50, 1000, 10 ** 6, 100
+    q = np.ones((1, 600, 3))
+    prob, fin = run(cfg, 3, q, [0], 200, 1)
+    assert prob == 0
+    assert fin == (200 - 48) + 1 + 1
+
+
+def test_random_batches():
+    """120 random batches: both lattices, chunks of 2-202 steps replayed up
+    to 7 times, 1-29 instances joining at random chunk boundaries"""
+    bad = []
+    for seed in range(120):
+        rng = np.random.default_rng(1000 + seed)
+        lattice = 3 if seed % 2 else 2
+        chunk = int(rng.choice([2, 4, 6, 10, 40, 200, 202]))
+        replays = int(rng.choice([1, 2, 3, 7]))
+        n, nsteps = int(rng.integers(1, 30)), 3000
+        q = flux_sequences(rng, n, nsteps, lattice)
+        cfgs = configs3d(rng, n) if lattice == 3 else configs2d(rng, n)
+        rnd = chunk * replays
+        start = rng.integers(0, max(1, nsteps // rnd // 2), size=n) * rnd
+        prob, _ = run(cfgs, lattice, q, start, chunk, replays)
+        if prob:
+            bad.append((seed, prob))
+    assert not bad, bad

@@ -62,6 +62,21 @@ def test_sparse_finish_bitwise_max_steps_3d(gpu_ctx):
     assert np.array_equal(a["velocity"], b["velocity"])
 
 
+def test_sparse_finish_bitwise_hold_zero(gpu_ctx):
+    """--hold 0: the first check (step 37, long before min_steps) ends the
+    first phase in the middle of a chunk"""
+    from flowperm import masks, solver
+    m = masks.spheres((24, 20, 28), 0.6, 2, 4, seed=9)
+    cfg = dict(ff=40, eps=1e-6, lag=37, hold=0, min_steps=1000, avg_steps=150, max_steps=3000,
+               precision="f32")
+    print("3d hold 0", flush=True)
+    a, b = _both(lambda: solver.solve_permeability3d(m, ctx=gpu_ctx, velocity=True, **cfg))
+    for k in KEYS3:
+        assert a[k] == b[k], k
+    assert a["steps_fx"] == 37 + 150
+    assert np.array_equal(a["velocity"], b["velocity"])
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32", "f32ref"])
 def test_sparse_finish_bitwise_2d(gpu_ctx, precision):
     from flowperm import masks, solver
