@@ -13,7 +13,11 @@ Timing: after the warmup steps, the K timed steps run twice.  First one kernel
 at a time with a HIP event pair around each (the per-kernel averages the
 roofline uses); then, after one untimed chunk, as replays of the captured
 200-step hipGraph chunk, exactly as solves run: value and ms_per_step come
-from this second pass (eager_ms_per_step reports the first).
+from this second pass (eager_ms_per_step reports the first).  The convergence
+rules are those of a first phase that never converges (eps 0, a check every
+2000 steps), so a chunk launches the finish kernel (flux sum and convergence
+update) only around its checks, as a solve's chunks do between checks;
+fp32_flux_every_step times the averaging phase, a finish after every step.
 
 Printed on one JSON line (stdout): the metric, the roofline of the step
 kernels (algorithmic bytes 2*19*4 = 152 B per fluid update over their average
