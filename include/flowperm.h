@@ -312,7 +312,11 @@ typedef struct fp_bench_out {
     int32_t deferred;         /* 1: total_ms ran with the finish of each    */
                               /* step deferred into the next step kernel    */
                               /* (lattices of at most 1024 work items per   */
-                              /* step kernel), one finish launch per chunk  */
+                              /* step kernel), one finish launch per chunk; */
+                              /* 0: finish launches, after every step or    */
+                              /* only where the chunk's finish plan puts    */
+                              /* them (a plan is taken when it keeps at     */
+                              /* most one finish per 5 steps)               */
     int32_t pad;
 } fp_bench_out;
 
