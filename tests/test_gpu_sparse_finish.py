@@ -156,3 +156,51 @@ def test_sparse_finish_bitwise_staggered_server():
     for ra, rb in zip(a, b):
         for k in KEYS3:
             assert ra[k] == rb[k], k
+
+
+def test_random_configs_bitwise(gpu_ctx):
+    """random convergence flags (lag, hold -1..3, min/max/avg steps, 2D
+    sample interval and tolerances) on small lattices, 3D and 2D: the default
+    settings (finish plans and the deferred finish, per chunk) against a
+    finish after every step, bit for bit"""
+    from flowperm import masks, solver
+    rng = np.random.default_rng(2024)
+    m3 = masks.spheres((20, 16, 24), 0.6, 2, 4, seed=11)
+    m2 = masks.ellipses((40, 48), 0.6, rmin=1.5, rmax=3.0, seed=12, axis=(1, 2))
+    dense = {"FLOWPERM_DEFER_FINISH": "0", "FLOWPERM_SPARSE_FINISH": "0"}
+    for case in range(12):
+        lag = int(rng.choice([7, 37, 100, 250, 1000]))
+        common = dict(lag=lag, hold=int(rng.choice([-1, 0, 1, 2, 3])),
+                      min_steps=int(rng.choice([0, 150, 600])),
+                      max_steps=int(rng.choice([90, 801, 2500])),
+                      avg_steps=int(rng.choice([0, 1, 77, 300])))
+        if case % 2 == 0:
+            cfg = dict(common, ff=40, eps=float(rng.choice([0.0, 1e-6, 1e-3])),
+                       precision=str(rng.choice(["f32", "f64"])))
+
+            def run():
+                return solver.solve_permeability3d(m3, ctx=gpu_ctx, velocity=True, **cfg)
+            keys = KEYS3
+        else:
+            cfg = dict(common, eps=float(rng.choice([0.0, 1e-3, 1e-2])),
+                       ksigma=float(rng.choice([0.0, 3.0])), qfloor=float(rng.choice([0.0, 1e-2])),
+                       avg_sample=int(rng.choice([1, 7, 50])),
+                       precision=str(rng.choice(["f32", "f32ref", "f64"])))
+
+            def run():
+                return solver.solve_permeability2d(m2, ctx=gpu_ctx, velocity=True, **cfg)
+            keys = KEYS2
+        print(f"random case {case}: {cfg}", flush=True)
+        a = run()
+        old = {k: os.environ.pop(k, None) for k in dense}
+        try:
+            os.environ.update(dense)
+            b = run()
+        finally:
+            for k, v in old.items():
+                os.environ.pop(k, None)
+                if v is not None:
+                    os.environ[k] = v
+        for k in keys:
+            assert a[k] == b[k], (case, k, a[k], b[k])
+        assert np.array_equal(a["velocity"], b["velocity"]), case
