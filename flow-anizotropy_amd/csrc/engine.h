@@ -20,8 +20,9 @@ void kern_step(int lattice, int prec, bool local, const Tuning &tu, int ad, cons
                const InstState *st, const void *work, int nwork, Partial *part, hipStream_t s,
                const Defer *defer = nullptr);
 // plain copies of n16 float4: kind 0 persistent grid, 1 one workgroup per
-// 8 KiB, 2 the same with non-temporal loads and stores
-constexpr int N_COPY_KINDS = 3;
+// 8 KiB, 2 the same with non-temporal loads and stores, 3 one workgroup per
+// 4 KiB, non-temporal
+constexpr int N_COPY_KINDS = 4;
 void kern_copy(int kind, const void *a, void *b, int64_t n16, hipStream_t s);
 // after a step: exact flux sums and the convergence update of every instance;
 // adv > 1 also counts the adv - 1 quiet steps before it (quiet_run_step)

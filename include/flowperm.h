@@ -477,9 +477,10 @@ void       fp_server_get_stats(fp_server *s, fp_server_stats *out);
 void fp_test_fail_rebuilds(int n);
 
 /* Plain device-to-device copy of `bytes` (read + write), `iters` times with
- * each of three copy kernels (a persistent grid; one workgroup per 8 KiB; the
- * same with non-temporal loads and stores): *ms_per_copy is the fastest, the
- * measured bandwidth the step kernels are compared with. */
+ * each of four copy kernels (a persistent grid; one workgroup per 8 KiB; the
+ * same with non-temporal loads and stores; one workgroup per 4 KiB,
+ * non-temporal): *ms_per_copy is the fastest, the measured bandwidth the step
+ * kernels are compared with. */
 int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_per_copy);
 
 #ifdef __cplusplus
