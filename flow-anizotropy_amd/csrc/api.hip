@@ -528,6 +528,15 @@ double fp_field_residual2d(int lx, int ly, const float *cur, const float *snap)
 
 void fp_test_fail_rebuilds(int n) { test_fail_rebuilds(n); }
 
+// the launch shapes of an fp_tuning (nbr_segs 0: the default, 4)
+static void copy_tuning(const fp_tuning *t, Tuning &tu)
+{
+    tu.local_cpt = t->local_cpt; tu.nbr_waves = t->nbr_waves;
+    tu.nontemporal = t->nontemporal; tu.local_bpc = t->local_bpc;
+    tu.nbr_bpc = t->nbr_bpc; tu.nbr_mode = t->nbr_mode;
+    tu.nbr_segs = t->nbr_segs > 0 ? t->nbr_segs : Tuning().nbr_segs;
+}
+
 int fp_verify_batch_tables(const uint8_t *const *solids, const int32_t *dims, int n, int lattice,
                            int precision, const fp_tuning *tune, int64_t *problems)
 {
@@ -535,11 +544,7 @@ int fp_verify_batch_tables(const uint8_t *const *solids, const int32_t *dims, in
        precision < FP_F64 || precision > FP_F32_REF2D)
         return FP_ERR_ARG;
     Tuning tu;
-    if(tune) {
-        tu.local_cpt = tune->local_cpt; tu.nbr_waves = tune->nbr_waves;
-        tu.nontemporal = tune->nontemporal; tu.local_bpc = tune->local_bpc;
-        tu.nbr_bpc = tune->nbr_bpc; tu.nbr_mode = tune->nbr_mode;
-    }
+    if(tune) copy_tuning(tune, tu);
     std::vector<const uint8_t *> s(solids, solids + n);
     std::vector<int> d(dims, dims + 3 * n);
     for(const uint8_t *p : s) if(!p) return FP_ERR_ARG;
@@ -955,18 +960,15 @@ static bool tuning_ok(const fp_tuning *t)
            t->nbr_waves >= 1 && t->nbr_waves <= 8 && t->chunk_steps >= 2 &&
            (t->nontemporal == 0 || t->nontemporal == 1) &&
            t->local_bpc >= 0 && t->local_bpc <= 32 && t->nbr_bpc >= 0 && t->nbr_bpc <= 32 &&
-           (t->nbr_mode >= 0 && t->nbr_mode <= 2);
+           (t->nbr_mode >= 0 && t->nbr_mode <= 2) &&
+           (t->nbr_segs == 0 || t->nbr_segs == 1 || t->nbr_segs == 2 || t->nbr_segs == 4 ||
+            t->nbr_segs == 8);
 }
 
 extern "C" int fp_set_tuning(fp_ctx *ctx, const fp_tuning *t)
 {
     if(!ctx || !tuning_ok(t)) return FP_ERR_ARG;
-    ctx->tune.local_cpt = t->local_cpt;
-    ctx->tune.nbr_waves = t->nbr_waves;
-    ctx->tune.nontemporal = t->nontemporal;
-    ctx->tune.local_bpc = t->local_bpc;
-    ctx->tune.nbr_bpc = t->nbr_bpc;
-    ctx->tune.nbr_mode = t->nbr_mode;
+    copy_tuning(t, ctx->tune);
     ctx->chunk = t->chunk_steps & ~1;
     return FP_OK;
 }
@@ -981,6 +983,7 @@ extern "C" void fp_get_tuning(fp_ctx *ctx, fp_tuning *t)
     t->local_bpc = ctx->tune.local_bpc;
     t->nbr_bpc = ctx->tune.nbr_bpc;
     t->nbr_mode = ctx->tune.nbr_mode;
+    t->nbr_segs = ctx->tune.nbr_segs;
 }
 
 extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
@@ -1017,10 +1020,7 @@ extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny,
     }
     Batch b(lattice, precision, ctx->stream, 2, false);
     Tuning tu = ctx->tune;
-    if(tune) {
-        tu.local_cpt = tune->local_cpt; tu.nbr_waves = tune->nbr_waves; tu.nontemporal = tune->nontemporal;
-        tu.local_bpc = tune->local_bpc; tu.nbr_bpc = tune->nbr_bpc; tu.nbr_mode = tune->nbr_mode;
-    }
+    if(tune) copy_tuning(tune, tu);
     b.set_tuning(tu);
     e = b.admit(&j);
     if(!e.empty()) return fail(ctx, e.find("memory") != std::string::npos ? FP_ERR_OOM : FP_ERR_DEVICE, e);
@@ -1095,15 +1095,18 @@ static const int TUNE_LOCAL[24][3] = {
     {4, 0, 0}, {4, 2, 0}, {4, 8, 0}, {4, 16, 0}, {1, 0, 1}, {1, 2, 1}, {1, 8, 1}, {1, 16, 1},
     {2, 0, 1}, {2, 2, 1}, {2, 8, 1}, {2, 16, 1}, {4, 0, 1}, {4, 2, 1}, {4, 8, 1}, {4, 16, 1}};
 // neighbour step: segments per workgroup (row-segment form), blocks/CU (0:
-// one per item), form (0 row segments, 1 compact)
-static const int TUNE_NBR[FP_TUNE_NNBR][3] = {
-    {2, 0, 0}, {2, 8, 0}, {4, 0, 0}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {8, 0, 0}, {8, 2, 0},
-    {8, 3, 0}, {8, 4, 0}, {1, 0, 1}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {1, 16, 1}, {1, 32, 1}};
+// one per item), form (0 row segments, 1 compact), x-adjacent segments of a
+// row among them (nbr_segs)
+static const int TUNE_NBR[FP_TUNE_NNBR][4] = {
+    {4, 0, 0, 4}, {8, 0, 0, 4}, {8, 0, 0, 8}, {2, 0, 0, 2}, {4, 0, 0, 2}, {8, 0, 0, 2},
+    {2, 0, 0, 1}, {4, 0, 0, 1}, {8, 0, 0, 1}, {4, 8, 0, 4}, {4, 4, 0, 1},
+    {1, 0, 1, 1}, {1, 2, 1, 1}, {1, 4, 1, 1}, {1, 8, 1, 1}, {1, 16, 1, 1}};
 
 extern "C" void fp_tune_shape(int kind, int i, int *a, int *b, int *c)
 {
     if(kind == 0 && i >= 0 && i < 24) { *a = TUNE_LOCAL[i][0]; *b = TUNE_LOCAL[i][1]; *c = TUNE_LOCAL[i][2]; }
     else if(kind == 1 && i >= 0 && i < FP_TUNE_NNBR) { *a = TUNE_NBR[i][0]; *b = TUNE_NBR[i][1]; *c = TUNE_NBR[i][2]; }
+    else if(kind == 2 && i >= 0 && i < FP_TUNE_NNBR) { *a = TUNE_NBR[i][3]; *b = *c = 0; }
     else { *a = *b = *c = 0; }
 }
 
@@ -1145,6 +1148,7 @@ extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, in
         Tuning t = ctx->tune;
         t.local_cpt = TUNE_LOCAL[i][0]; t.local_bpc = TUNE_LOCAL[i][1]; t.nontemporal = TUNE_LOCAL[i][2];
         t.nbr_waves = TUNE_NBR[n][0]; t.nbr_bpc = TUNE_NBR[n][1]; t.nbr_mode = TUNE_NBR[n][2];
+        t.nbr_segs = TUNE_NBR[n][3];
         b.set_tuning(t);
         Batch::KernelTimes kw, kt;
         e = b.time_steps(2, kw);
@@ -1167,6 +1171,7 @@ extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, in
     best->nbr_waves = TUNE_NBR[bn][0];
     best->nbr_bpc = TUNE_NBR[bn][1];
     best->nbr_mode = TUNE_NBR[bn][2];
+    best->nbr_segs = TUNE_NBR[bn][3];
     best->chunk_steps = ctx->chunk;
     if(local_ms) for(int i = 0; i < FP_TUNE_NLOCAL; i++) local_ms[i] = lt[i];
     if(nbr_ms) for(int i = 0; i < FP_TUNE_NNBR; i++) nbr_ms[i] = nt[i];

@@ -233,7 +233,9 @@ struct BatchTables {
     std::vector<RedWork> lr, nr;         // reduction items over local / neighbour partials
 };
 void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const Tuning &tu, BatchTables &t);
-int64_t check_tables(const BatchTables &t, std::string *first);
+// the check (number of violations, the first in *first); nbr_waves: the
+// wavefronts of a neighbour workgroup
+int64_t check_tables(const BatchTables &t, std::string *first, int nbr_waves);
 // host only: the tables of the structures' forcing runs as a batch would build
 // them, checked (*bad violations); returns the first problem, or a mask error
 std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,

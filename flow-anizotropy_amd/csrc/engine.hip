@@ -582,7 +582,11 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
         }
         desc[i].np0 = (int32_t)nw.size();
         {
-            // slabs of nbr_waves rows stacked along z (2D: y); inside a slab y
+            // Items of S x-adjacent segments of nbr_waves / S rows stacked
+            // along z (2D: y).  A row's segments in one workgroup share the
+            // lines at their boundaries in its L1: at 256^3 FP32 a whole row
+            // per workgroup (S = 4) takes the neighbour step from 0.382 to
+            // 0.345 ms against 4 z-stacked segments (S = 1).  Inside a slab y
             // then the segment vary fastest.  The kernels split their items
             // into 8 contiguous ranges, one per XCD.  Large lattices deal the
             // rows in 8 bands of ny/8 rows, each band through all z, slab after
@@ -594,7 +598,13 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
             // range then covers the same cells as its local-step range, and
             // the L2 lines the one kernel leaves are the other's (192^3 FP32:
             // bands 0.139, z order 0.135 ms; 96^3: 0.030 against 0.027).
-            const int64_t nwv = tu.nbr_waves;
+            // S = 2^sh x-adjacent segments per row (the largest power of two
+            // within nbr_segs, nbr_waves and the row's segments) and
+            // nbr_waves / S rows per item
+            const int cap = std::min(tu.nbr_segs, tu.nbr_waves);
+            int sh = 0;
+            while((2 << sh) <= cap && (2 << sh) <= g.nseg) sh++;
+            const int64_t nwv = std::max(1, tu.nbr_waves >> sh);
             const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
             const int64_t ys = g.dim == 3 ? g.ny : 1;
             const int64_t band = banded ? (ys + 7) / 8 : ys;
@@ -602,10 +612,11 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
             for(int64_t p0 = 0; p0 < planes; p0 += nwv) {
                 const int nu = (int)std::min<int64_t>(nwv, planes - p0);
                 for(int64_t y = y0; y < std::min<int64_t>(ys, y0 + band); y++)
-                    for(int sg = 0; sg < g.nseg; sg++) {
+                    for(int sg = 0; sg < g.nseg; sg += 1 << sh) {
                         NbrWork w;
                         w.inst = i;
-                        w.nunit = (int16_t)nu;
+                        w.nunit = (uint8_t)nu;
+                        w.sshift = (uint8_t)sh;
                         w.seg = (int16_t)sg;
                         w.y = (int32_t)y;
                         w.p0 = (int32_t)p0;
@@ -652,10 +663,41 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
 // items tile its range in order, each at most REDUCE_SPAN slots, and are the
 // ones its InstDesc names.  Returns the number of violations (the first one
 // described in *first).
-int64_t check_tables(const BatchTables &t, std::string *first)
+int64_t check_tables(const BatchTables &t, std::string *first, int nbr_waves)
 {
     int64_t bad = 0;
     auto fail = [&](const std::string &m) { if(!bad && first) *first = m; bad++; };
+    // every (row, segment) unit of a row-segment instance in exactly one
+    // wavefront of one item, whose workgroup has that wavefront
+    {
+        std::vector<std::vector<uint8_t>> seen(t.desc.size());
+        for(size_t it = 0; it < t.nw.size(); it++) {
+            const NbrWork &w = t.nw[it];
+            if(w.inst < 0 || w.inst >= (int32_t)t.desc.size()) continue;          // reported below
+            const InstDesc &D = t.desc[(size_t)w.inst];
+            const int64_t nrows = D.lattice == 3 ? (int64_t)D.ny * D.nz : D.ny;
+            std::vector<uint8_t> &sv = seen[(size_t)w.inst];
+            if(sv.empty()) sv.assign((size_t)(nrows * D.nseg), 0);
+            const int S = 1 << w.sshift;
+            if(((int)w.nunit << w.sshift) > nbr_waves || w.nunit < 1)
+                fail("row-segment item " + std::to_string(it) + " needs more wavefronts than a workgroup has");
+            for(int dz = 0; dz < w.nunit; dz++)
+                for(int dsg = 0; dsg < S; dsg++) {
+                    const int64_t sg = w.seg + dsg, p = w.p0 + dz;
+                    if(sg >= D.nseg) continue;                                   // an idle wavefront
+                    const int64_t row = D.lattice == 3 ? p * D.ny + w.y : p;
+                    if(row < 0 || row >= nrows || sg < 0 || (D.lattice == 3 && (w.y < 0 || w.y >= D.ny))) {
+                        fail("row-segment item " + std::to_string(it) + " outside its lattice");
+                        continue;
+                    }
+                    if(sv[(size_t)(row * D.nseg + sg)]++)
+                        fail("unit of row " + std::to_string(row) + " segment " + std::to_string(sg) + " stepped twice");
+                }
+        }
+        for(size_t i = 0; i < seen.size(); i++)
+            for(size_t u = 0; u < seen[i].size(); u++)
+                if(!seen[i][u]) { fail("instance " + std::to_string(i) + " unit " + std::to_string(u) + " in no item"); break; }
+    }
     const int64_t n_inst = (int64_t)t.desc.size();
     const int64_t nl = (int64_t)t.lw.size(), nn = (int64_t)t.nw.size() + (int64_t)t.cw.size();
     auto in_inst = [&](int64_t slot, int32_t inst, bool local) {
@@ -740,7 +782,7 @@ std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,
     BatchTables t;
     build_tables(ptrs, lattice, prec, tu, t);
     std::string first;
-    *bad = check_tables(t, &first);
+    *bad = check_tables(t, &first, tu.nbr_waves);
     return first;
 }
 

@@ -633,7 +633,10 @@ struct Defer {
 struct Tuning {
     int local_cpt = 2;      // cells per thread in the local kernel: 1, 2 or 4
     int nbr_waves = 4;      // wavefronts per workgroup in the neighbour kernel: 1..8, one
-                            // row segment each, stacked along z (along y in 2D)
+                            // row segment each
+    int nbr_segs = 4;       // of those, x-adjacent segments of one row (1, 2, 4 or 8, at
+                            // most nbr_waves and the row's segments); the rest of the
+                            // workgroup's wavefronts stack rows along z (along y in 2D)
     int nontemporal = 1;    // local kernel: population loads/stores bypass cache allocation
     int local_bpc = 0;      // resident workgroups per CU launched (each loops over items);
                             // 0: one workgroup per work item
@@ -663,12 +666,15 @@ struct LocalWork {
     int64_t first;
 };
 
-// One neighbour-step work item: nunit row segments stacked along the slowest
-// axis, wave w taking the one at plane p0 + w.  3D: rows (y, z = p0 + w);
-// 2D: rows y = p0 + w.  seg is the 64-cell segment of the row.
+// One neighbour-step work item: S = 2^sshift x-adjacent 64-cell segments
+// (seg .. seg + S - 1) of each of nunit rows stacked along the slowest axis.
+// Wave w takes segment seg + (w mod S) of the row at plane p0 + w / S.  3D:
+// rows (y, z = p0 + w / S); 2D: rows y = p0 + w / S.  Segments past the row's
+// end leave their wave idle.
 struct NbrWork {
     int32_t inst;
-    int16_t nunit;
+    uint8_t nunit;
+    uint8_t sshift;
     int16_t seg;
     int32_t y;
     int32_t p0;

@@ -21,13 +21,13 @@ from . import _lib
 
 
 FIELDS = ("local_cpt", "nbr_waves", "chunk_steps", "nontemporal", "local_bpc", "nbr_bpc",
-          "nbr_mode")
+          "nbr_mode", "nbr_segs")
 N_LOCAL, N_NBR = 24, 16              # FP_TUNE_NLOCAL, FP_TUNE_NNBR
 
 
 def shapes(lib):
     """The shapes fp_autotune tries: (local list of (cpt, bpc, nt), neighbour
-    list of (waves, bpc, mode))."""
+    list of (waves, bpc, mode, segs))."""
     a, b, c = C.c_int(), C.c_int(), C.c_int()
     loc, nbr = [], []
     for i in range(N_LOCAL):
@@ -35,12 +35,14 @@ def shapes(lib):
         loc.append((a.value, b.value, c.value))
     for i in range(N_NBR):
         lib.fp_tune_shape(1, i, C.byref(a), C.byref(b), C.byref(c))
-        nbr.append((a.value, b.value, c.value))
+        w, bb, m = a.value, b.value, c.value
+        lib.fp_tune_shape(2, i, C.byref(a), C.byref(b), C.byref(c))
+        nbr.append((w, bb, m, a.value))
     return loc, nbr
 
 
-def nbr_name(w, b, m):
-    return f"compact_bpc{b}" if m == 1 else f"waves{w}_bpc{b}"
+def nbr_name(w, b, m, s=1):
+    return f"compact_bpc{b}" if m == 1 else f"waves{w}_segs{s}_bpc{b}"
 
 
 def cache_path() -> Path:
@@ -65,7 +67,8 @@ def device_name() -> str:
 def _key(lattice, precision, n_cells, porosity):
     # the best neighbour-step form depends on how sparse the lattice is
     size_class = "small" if n_cells < 2_000_000 else "large"
-    return f"{device_name()}/{lattice}/{precision}/{size_class}/phi{round(porosity, 1):.1f}"
+    # (the neighbour shapes gained nbr_segs: entries of older libraries lack it)
+    return f"{device_name()}/{lattice}/{precision}/{size_class}/phi{round(porosity, 1):.1f}/s"
 
 
 def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
@@ -97,7 +100,7 @@ def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
     rec = {k: getattr(best, k) for k in FIELDS}
     loc, nbr = shapes(ctx.lib)
     rec["local_ms"] = {f"cpt{c}_bpc{b}_nt{n}": round(v, 5) for (c, b, n), v in zip(loc, lms)}
-    rec["nbr_ms"] = {nbr_name(w, b, m): round(v, 5) for (w, b, m), v in zip(nbr, nms)
+    rec["nbr_ms"] = {nbr_name(w, b, m, sg): round(v, 5) for (w, b, m, sg), v in zip(nbr, nms)
                      if v < 1e29}
     if use_cache:
         cache[key] = rec

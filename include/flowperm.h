@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 8
+#define FP_ABI_VERSION 9
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -322,10 +322,11 @@ typedef struct fp_bench_out {
 
 /* Launch shapes of the step kernels; the autotuner picks them per device.  A new
    context starts with local_cpt 2, nbr_waves 4, chunk_steps 200, nontemporal 1,
-   local_bpc 0, nbr_bpc 0, nbr_mode 2 (the shapes tuned for D3Q19 at 256^3). */
+   local_bpc 0, nbr_bpc 0, nbr_mode 2, nbr_segs 4 (the shapes tuned for D3Q19 at
+   256^3). */
 typedef struct fp_tuning {
     int32_t local_cpt;        /* cells per thread, local kernel: 1, 2 or 4            */
-    int32_t nbr_waves;        /* z-stacked row segments per workgroup, neighbour
+    int32_t nbr_waves;        /* row segments (wavefronts) per workgroup, neighbour
                                  kernel: 1..8                                         */
     int32_t chunk_steps;      /* steps per hipGraph replay (even)                     */
     int32_t nontemporal;      /* 1: local-kernel loads/stores do not allocate in cache */
@@ -336,6 +337,10 @@ typedef struct fp_tuning {
                                  segment, 1 = 64 fluid cells per wavefront (a 24-byte
                                  per-cell neighbour record; faster below porosity
                                  ~0.7), 2 = chosen per structure by porosity (default) */
+    int32_t nbr_segs;         /* of the nbr_waves segments, how many lie side by side
+                                 along x in one row: 1, 2, 4 or 8 (limited to
+                                 nbr_waves and to the row's segments; the others stack
+                                 rows along z, along y in 2D); 0 = 4 (default)        */
 } fp_tuning;
 
 /* Set / read the launch shapes a context uses for its solves and batches. */
@@ -391,7 +396,8 @@ int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, in
 int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
                 int precision, int steps, fp_tuning *best, double *local_ms, double *nbr_ms);
 /* The i-th shape tried by fp_autotune: kind 0 = local (cpt, bpc, nontemporal in
- * a, b, c), kind 1 = neighbour (waves, bpc, nbr_mode in a, b, c). */
+ * a, b, c), kind 1 = neighbour (waves, bpc, nbr_mode in a, b, c), kind 2 = the
+ * same neighbour shape's nbr_segs (in a). */
 void fp_tune_shape(int kind, int i, int *a, int *b, int *c);
 
 /* ------------------------------------------------------------------------ */

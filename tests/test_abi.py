@@ -147,12 +147,17 @@ def test_tune_shapes_are_valid(lib):
     for i in range(24):
         lib.fp_tune_shape(0, i, C.byref(a), C.byref(b), C.byref(c))
         assert a.value in (1, 2, 4) and 0 <= b.value <= 32 and c.value in (0, 1)
-    modes = set()
+    modes, segs = set(), set()
     for i in range(16):
         lib.fp_tune_shape(1, i, C.byref(a), C.byref(b), C.byref(c))
         assert 1 <= a.value <= 8 and 0 <= b.value <= 32 and c.value in (0, 1)
         modes.add(c.value)
+        waves = a.value
+        lib.fp_tune_shape(2, i, C.byref(a), C.byref(b), C.byref(c))
+        assert a.value in (1, 2, 4, 8) and a.value <= waves
+        segs.add(a.value)
     assert modes == {0, 1}                      # both neighbour-step forms are tried
+    assert {1, 2, 4} <= segs                    # and rows 1, 2 and 4 segments wide
 
 
 def test_now_is_monotonic(lib):

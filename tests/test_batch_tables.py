@@ -32,7 +32,7 @@ def verify(masks, lattice=3, precision=1, **tune):
     if tune:
         t = _lib.Tuning(tune.get("local_cpt", 2), tune.get("nbr_waves", 4), 200,
                         tune.get("nontemporal", 1), tune.get("local_bpc", 0),
-                        tune.get("nbr_bpc", 0), tune.get("nbr_mode", 2))
+                        tune.get("nbr_bpc", 0), tune.get("nbr_mode", 2), tune.get("nbr_segs", 0))
     bad = C.c_int64(-1)
     st = lib.fp_verify_batch_tables(ptrs, d, len(arrs), lattice, precision,
                                     C.byref(t) if t is not None else None, C.byref(bad))
@@ -51,7 +51,9 @@ def mixed_pair():
 
 @pytest.mark.parametrize("tune", [{}, {"nbr_waves": 1}, {"nbr_waves": 2}, {"nbr_waves": 8},
                                   {"local_cpt": 1}, {"local_cpt": 4}, {"nbr_mode": 0},
-                                  {"nbr_mode": 1}, {"nbr_bpc": 8, "local_bpc": 16}])
+                                  {"nbr_mode": 1}, {"nbr_bpc": 8, "local_bpc": 16},
+                                  {"nbr_segs": 1}, {"nbr_segs": 2}, {"nbr_waves": 8, "nbr_segs": 8},
+                                  {"nbr_waves": 6, "nbr_segs": 4}])
 def test_mixed_form_batch_layout(tune):
     assert verify(mixed_pair(), **tune) == 0
     assert verify(mixed_pair()[::-1], **tune) == 0
@@ -86,3 +88,23 @@ def test_2d_layout():
     ms = [masks.ellipses((64, 80), 0.6, seed=1), masks.ellipses((300, 257), 0.4, seed=2)]
     for mode in (0, 1, 2):
         assert verify(ms, lattice=2, precision=2, nbr_mode=mode) == 0
+
+
+@pytest.mark.parametrize("lattice", [3, 2])
+@pytest.mark.parametrize("tune", [{"nbr_segs": 1}, {"nbr_segs": 2}, {}, {"nbr_waves": 8, "nbr_segs": 8},
+                                  {"nbr_waves": 8, "nbr_segs": 4}, {"nbr_waves": 2, "nbr_segs": 4},
+                                  {"nbr_waves": 3, "nbr_segs": 2}, {"nbr_waves": 6, "nbr_segs": 4}])
+def test_row_segments_side_by_side_cover_every_unit_once(lattice, tune):
+    """items of S x-adjacent segments: rows of 1 to 9 segments (x 40 ... 530),
+    so that a row's last item has segments past its end (idle wavefronts),
+    and slabs whose last one is short; every (row, segment) unit is stepped by
+    exactly one wavefront of a workgroup that has it"""
+    from flowperm import masks
+    rng = np.random.default_rng(4)
+    if lattice == 3:
+        ms = [(rng.random((nz, ny, nx)) < 0.35).astype(np.uint8)
+              for nz, ny, nx in ((7, 5, 40), (9, 6, 130), (5, 3, 200), (6, 4, 530), (3, 3, 300))]
+    else:
+        ms = [(rng.random((ny, nx)) < 0.35).astype(np.uint8)
+              for ny, nx in ((13, 40), (11, 190), (7, 530), (5, 260))]
+    assert verify(ms, lattice=lattice, nbr_mode=0, **tune) == 0

@@ -1,0 +1,68 @@
+"""
+Neighbour items of S x-adjacent row segments (nbr_segs 1, 2, 4, 8): the same
+results bit for bit whatever S, on rows of 3 and 5 segments (so a row's last
+item has segments past the row's end) in 3D and 2D, FP32 and FP64, with the
+velocity fields.  The default (S = 4) is also what every other GPU test runs;
+the reference fixtures with rows of 70 and 130 cells pin it to the oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from test_gpu_deferred import KEYS2, KEYS3
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(waves, segs):
+    from flowperm import _lib, solver
+    ctx = solver.Context()
+    t = _lib.Tuning()
+    ctx.lib.fp_get_tuning(ctx.handle, C.byref(t))
+    t.nbr_mode, t.nbr_waves, t.nbr_segs = 0, waves, segs
+    assert ctx.lib.fp_set_tuning(ctx.handle, C.byref(t)) == 0
+    return ctx
+
+
+SHAPES = [(4, 1), (4, 2), (4, 4), (8, 8), (8, 4), (2, 2)]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_segments_per_item_bitwise_3d(precision):
+    from flowperm import masks, solver
+    m = masks.spheres((10, 12, 150), 0.6, 2, 4, seed=21)            # 3 segments per row
+    cfg = dict(ff=40, eps=1e-6, lag=100, min_steps=200, avg_steps=100, max_steps=600,
+               precision=precision)
+    res = []
+    for waves, segs in SHAPES:
+        print(f"3d {precision} waves {waves} segs {segs}", flush=True)
+        ctx = _ctx(waves, segs)
+        try:
+            res.append(solver.solve_permeability3d(m, ctx=ctx, velocity=True, **cfg))
+        finally:
+            ctx.close()
+    for r in res[1:]:
+        for k in KEYS3:
+            assert r[k] == res[0][k], k
+        assert np.array_equal(r["velocity"], res[0]["velocity"])
+
+
+@pytest.mark.parametrize("precision", ["f32", "f32ref", "f64"])
+def test_segments_per_item_bitwise_2d(precision):
+    from flowperm import masks, solver
+    m = masks.ellipses((40, 300), 0.6, rmin=1.5, rmax=3.0, seed=22, axis=(1, 2))   # 5 segments
+    cfg = dict(lag=150, min_steps=300, max_steps=900, avg_steps=200, avg_sample=10)
+    res = []
+    for waves, segs in SHAPES:
+        print(f"2d {precision} waves {waves} segs {segs}", flush=True)
+        ctx = _ctx(waves, segs)
+        try:
+            res.append(solver.solve_permeability2d(m, ctx=ctx, precision=precision, velocity=True,
+                                                   **cfg))
+        finally:
+            ctx.close()
+    for r in res[1:]:
+        for k in KEYS2:
+            assert r[k] == res[0][k], k
+        assert np.array_equal(r["velocity"], res[0]["velocity"])
