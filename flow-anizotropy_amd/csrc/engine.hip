@@ -615,9 +615,7 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
                     for(int sg = 0; sg < g.nseg; sg += 1 << sh) {
                         NbrWork w;
                         w.inst = i;
-                        w.nunit = (uint8_t)nu;
-                        w.sshift = (uint8_t)sh;
-                        w.seg = (int16_t)sg;
+                        w.set_shape(sg, nu, sh);
                         w.y = (int32_t)y;
                         w.p0 = (int32_t)p0;
                         nw.push_back(w);
@@ -678,12 +676,12 @@ int64_t check_tables(const BatchTables &t, std::string *first, int nbr_waves)
             const int64_t nrows = D.lattice == 3 ? (int64_t)D.ny * D.nz : D.ny;
             std::vector<uint8_t> &sv = seen[(size_t)w.inst];
             if(sv.empty()) sv.assign((size_t)(nrows * D.nseg), 0);
-            const int S = 1 << w.sshift;
-            if(((int)w.nunit << w.sshift) > nbr_waves || w.nunit < 1)
+            const int S = 1 << w.sshift();
+            if((w.nunit() << w.sshift()) > nbr_waves || w.nunit() < 1)
                 fail("row-segment item " + std::to_string(it) + " needs more wavefronts than a workgroup has");
-            for(int dz = 0; dz < w.nunit; dz++)
+            for(int dz = 0; dz < w.nunit(); dz++)
                 for(int dsg = 0; dsg < S; dsg++) {
-                    const int64_t sg = w.seg + dsg, p = w.p0 + dz;
+                    const int64_t sg = w.seg() + dsg, p = w.p0 + dz;
                     if(sg >= D.nseg) continue;                                   // an idle wavefront
                     const int64_t row = D.lattice == 3 ? p * D.ny + w.y : p;
                     if(row < 0 || row >= nrows || sg < 0 || (D.lattice == 3 && (w.y < 0 || w.y >= D.ny))) {

@@ -524,11 +524,11 @@ k_nbr(const InstDesc *__restrict__ insts, const InstState *__restrict__ st,
         cptr<SegInfo> seg = as_const<SegInfo>(D.seg);
 
         // wave -> (segment seg + dsg, row p0 + dz) of the item's S x nunit units
-        const int sh = w.sshift;
+        const int sh = w.sshift(), seg0 = w.seg();
         const int dsg = wave & ((1 << sh) - 1), dz = wave >> sh;
-        const bool wave_act = dz < w.nunit && w.seg + dsg < nseg;
+        const bool wave_act = dz < w.nunit() && seg0 + dsg < nseg;
         const int p = uni(w.p0 + (wave_act ? dz : 0));
-        const int sg = uni(w.seg + (wave_act ? dsg : 0));
+        const int sg = uni(seg0 + (wave_act ? dsg : 0));
         const int row = (L::DIM == 3) ? p * ny + w.y : p;
         const int unit = row * nseg + sg;
 
