@@ -599,11 +599,13 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
             // the L2 lines the one kernel leaves are the other's (192^3 FP32:
             // bands 0.139, z order 0.135 ms; 96^3: 0.030 against 0.027).
             // S = 2^sh x-adjacent segments per row (the largest power of two
-            // within nbr_segs, nbr_waves and the row's segments) and
-            // nbr_waves / S rows per item
+            // within nbr_segs and nbr_waves that divides the row's segments:
+            // an item with segments past the row's end idles wavefronts, and
+            // rows of 3 segments took 0.140 ms with S = 2 against 0.111 with
+            // S = 1 at 160^3) and nbr_waves / S rows per item
             const int cap = std::min(tu.nbr_segs, tu.nbr_waves);
             int sh = 0;
-            while((2 << sh) <= cap && (2 << sh) <= g.nseg) sh++;
+            while((2 << sh) <= cap && g.nseg % (2 << sh) == 0) sh++;
             const int64_t nwv = std::max(1, tu.nbr_waves >> sh);
             const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
             const int64_t ys = g.dim == 3 ? g.ny : 1;

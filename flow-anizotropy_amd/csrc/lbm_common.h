@@ -635,8 +635,9 @@ struct Tuning {
     int nbr_waves = 4;      // wavefronts per workgroup in the neighbour kernel: 1..8, one
                             // row segment each
     int nbr_segs = 4;       // of those, x-adjacent segments of one row (1, 2, 4 or 8, at
-                            // most nbr_waves and the row's segments); the rest of the
-                            // workgroup's wavefronts stack rows along z (along y in 2D)
+                            // most nbr_waves, and a divisor of the row's segments); the
+                            // rest of the workgroup's wavefronts stack rows along z
+                            // (along y in 2D)
     int nontemporal = 1;    // local kernel: population loads/stores bypass cache allocation
     int local_bpc = 0;      // resident workgroups per CU launched (each loops over items);
                             // 0: one workgroup per work item

@@ -338,9 +338,10 @@ typedef struct fp_tuning {
                                  per-cell neighbour record; faster below porosity
                                  ~0.7), 2 = chosen per structure by porosity (default) */
     int32_t nbr_segs;         /* of the nbr_waves segments, how many lie side by side
-                                 along x in one row: 1, 2, 4 or 8 (limited to
-                                 nbr_waves and to the row's segments; the others stack
-                                 rows along z, along y in 2D); 0 = 4 (default)        */
+                                 along x in one row: 1, 2, 4 or 8 (at most nbr_waves,
+                                 and lowered to a divisor of the row's 64-cell
+                                 segments; the others stack rows along z, along y in
+                                 2D); 0 = 4 (default)                                 */
 } fp_tuning;
 
 /* Set / read the launch shapes a context uses for its solves and batches. */

@@ -1,9 +1,10 @@
 """
 Neighbour items of S x-adjacent row segments (nbr_segs 1, 2, 4, 8): the same
-results bit for bit whatever S, on rows of 3 and 5 segments (so a row's last
-item has segments past the row's end) in 3D and 2D, FP32 and FP64, with the
-velocity fields.  The default (S = 4) is also what every other GPU test runs;
-the reference fixtures with rows of 70 and 130 cells pin it to the oracle.
+results bit for bit whatever S and however many rows an item stacks, on rows
+of 4 segments (3D) and 8 segments (2D), whose segment counts every S tried
+divides, in FP32 and FP64, with the velocity fields.  The default (S = 4) is
+also what every other GPU test runs; the reference fixtures with rows of 70
+and 130 cells (S = 2 and 1) pin it to the oracle.
 """
 import ctypes as C
 
@@ -31,7 +32,7 @@ SHAPES = [(4, 1), (4, 2), (4, 4), (8, 8), (8, 4), (2, 2)]
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_segments_per_item_bitwise_3d(precision):
     from flowperm import masks, solver
-    m = masks.spheres((10, 12, 150), 0.6, 2, 4, seed=21)            # 3 segments per row
+    m = masks.spheres((10, 12, 256), 0.6, 2, 4, seed=21)            # 4 segments per row
     cfg = dict(ff=40, eps=1e-6, lag=100, min_steps=200, avg_steps=100, max_steps=600,
                precision=precision)
     res = []
@@ -51,7 +52,7 @@ def test_segments_per_item_bitwise_3d(precision):
 @pytest.mark.parametrize("precision", ["f32", "f32ref", "f64"])
 def test_segments_per_item_bitwise_2d(precision):
     from flowperm import masks, solver
-    m = masks.ellipses((40, 300), 0.6, rmin=1.5, rmax=3.0, seed=22, axis=(1, 2))   # 5 segments
+    m = masks.ellipses((40, 512), 0.6, rmin=1.5, rmax=3.0, seed=22, axis=(1, 2))   # 8 segments
     cfg = dict(lag=150, min_steps=300, max_steps=900, avg_steps=200, avg_sample=10)
     res = []
     for waves, segs in SHAPES:
