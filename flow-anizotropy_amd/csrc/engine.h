@@ -233,6 +233,10 @@ struct BatchTables {
     std::vector<RedWork> lr, nr;         // reduction items over local / neighbour partials
 };
 void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const Tuning &tu, BatchTables &t);
+// x-adjacent row segments per row-segment work item for rows of nseg
+// segments: the largest power of two within nbr_segs and nbr_waves that
+// divides nseg
+int segs_per_item(const Tuning &tu, int64_t nseg);
 // the check (number of violations, the first in *first); nbr_waves: the
 // wavefronts of a neighbour workgroup
 int64_t check_tables(const BatchTables &t, std::string *first, int nbr_waves);

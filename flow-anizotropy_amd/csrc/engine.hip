@@ -598,14 +598,10 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
             // range then covers the same cells as its local-step range, and
             // the L2 lines the one kernel leaves are the other's (192^3 FP32:
             // bands 0.139, z order 0.135 ms; 96^3: 0.030 against 0.027).
-            // S = 2^sh x-adjacent segments per row (the largest power of two
-            // within nbr_segs and nbr_waves that divides the row's segments:
-            // an item with segments past the row's end idles wavefronts, and
-            // rows of 3 segments took 0.140 ms with S = 2 against 0.111 with
-            // S = 1 at 160^3) and nbr_waves / S rows per item
-            const int cap = std::min(tu.nbr_segs, tu.nbr_waves);
+            // S = 2^sh x-adjacent segments per row (segs_per_item) and
+            // nbr_waves / S rows per item
             int sh = 0;
-            while((2 << sh) <= cap && g.nseg % (2 << sh) == 0) sh++;
+            while((2 << sh) <= segs_per_item(tu, g.nseg)) sh++;
             const int64_t nwv = std::max(1, tu.nbr_waves >> sh);
             const int64_t planes = g.dim == 3 ? g.nz : g.ny;       // stacking axis
             const int64_t ys = g.dim == 3 ? g.ny : 1;
@@ -663,6 +659,16 @@ void build_tables(const std::vector<Job *> &jobs, int lattice, int prec, const T
 // items tile its range in order, each at most REDUCE_SPAN slots, and are the
 // ones its InstDesc names.  Returns the number of violations (the first one
 // described in *first).
+int segs_per_item(const Tuning &tu, int64_t nseg)
+{
+    // an item with segments past the row's end idles wavefronts: rows of 3
+    // segments took 0.140 ms with S = 2 against 0.111 with S = 1 at 160^3
+    const int cap = std::min(tu.nbr_segs, tu.nbr_waves);
+    int s = 1;
+    while(2 * s <= cap && nseg % (2 * s) == 0) s *= 2;
+    return s;
+}
+
 int64_t check_tables(const BatchTables &t, std::string *first, int nbr_waves)
 {
     int64_t bad = 0;
@@ -771,7 +777,9 @@ std::string verify_batch_tables(const std::vector<const uint8_t *> &solids,
             j->prec = prec;
             j->stride = pop_stride(g->n_fluid);
             int m = tu.nbr_mode;
-            if(m == 2) m = (double)g->n_fluid < COMPACT_BELOW_POROSITY * (double)g->nn ? 1 : 0;
+            if(m == 2)
+                m = prefer_compact((double)g->n_fluid / (double)g->nn, segs_per_item(tu, g->nseg),
+                                   lattice, prec) ? 1 : 0;
             if(m == 1 && g->nx >= 0xFFFF) m = 0;
             j->nbr_mode = m;
             ptrs.push_back(j.get());
@@ -827,7 +835,9 @@ std::string Batch::rebuild()
     for(Job *j : jobs_) {
         const Geometry &g = *j->geo;
         int m = tune_.nbr_mode;
-        if(m == 2) m = (double)g.n_fluid < COMPACT_BELOW_POROSITY * (double)g.nn ? 1 : 0;
+        if(m == 2)
+            m = prefer_compact((double)g.n_fluid / (double)g.nn, segs_per_item(tune_, g.nseg),
+                               lattice_, prec_) ? 1 : 0;
         if(m == 1 && !geometry_build_compact(*j->geo).empty()) m = 0;
         j->nbr_mode = m;
         n_mode[m]++;

@@ -649,10 +649,29 @@ struct Tuning {
 
 // Below this porosity the compact neighbour step is the faster form: it keeps
 // every lane busy, while the row-segment step idles the lanes of solid cells.
-// Compact time over row-segment time on MI355X (D3Q19, 192^3 ellipsoids,
-// FP32 / FP64): 0.68 / 0.71 at porosity 0.30, 0.89 / 0.94 at 0.52, 0.94 / 0.97
-// at 0.62, 1.02 / 1.04 at 0.72, 1.05 / 1.06 at 0.92.
+// Compact time over row-segment time on MI355X (D3Q19, 192^3 ellipsoids, rows
+// of 3 segments, one per workgroup, FP32 / FP64): 0.68 / 0.71 at porosity 0.30,
+// 0.89 / 0.94 at 0.52, 0.94 / 0.97 at 0.62, 1.02 / 1.04 at 0.72, 1.05 / 1.06
+// at 0.92.
 constexpr double COMPACT_BELOW_POROSITY = 0.68;
+// D3Q19 rows whose row-segment workgroups take 4 or more of their segments
+// share the lines at the segment boundaries in one L1, and the crossover moves
+// down (256^3 ellipsoids, compact over row-segment time, FP32 / FP64: 0.88 /
+// 0.84 at porosity 0.40, 1.00 / 0.94 at 0.50, 1.05 / 0.98 at 0.58, 1.09 / 1.02
+// at 0.66, 1.09 / 1.03 at 0.75)
+constexpr double COMPACT_BELOW_POROSITY_WIDE_F32 = 0.50;
+constexpr double COMPACT_BELOW_POROSITY_WIDE_F64 = 0.62;
+
+// whether a structure of this porosity takes the compact form when the form
+// is chosen per structure (nbr_mode 2); segs: the row segments per
+// row-segment workgroup it would get
+inline bool prefer_compact(double porosity, int segs, int lattice, int prec)
+{
+    double below = COMPACT_BELOW_POROSITY;
+    if(lattice == 3 && segs >= 4)
+        below = prec == PREC_F64 ? COMPACT_BELOW_POROSITY_WIDE_F64 : COMPACT_BELOW_POROSITY_WIDE_F32;
+    return porosity < below;
+}
 
 // From this many bytes of populations a 3D instance deals its neighbour-step
 // items to the 8 XCDs in y bands through all z instead of z ranges (the slabs

@@ -59,10 +59,11 @@ namespace {
 // device memory a request holds while resident: its segment tables, the
 // compact neighbour table when its structure will use that form, and the
 // population (and, in 2D, velocity snapshot) arrays of each of its runs
-size_t request_bytes(const Geometry &g, int lattice, int prec, int njobs, int nbr_mode)
+size_t request_bytes(const Geometry &g, int lattice, int prec, int njobs, const Tuning &tu)
 {
-    const bool compact = nbr_mode == 1 ||
-                         (nbr_mode == 2 && (double)g.n_fluid < COMPACT_BELOW_POROSITY * (double)g.nn);
+    const bool compact = tu.nbr_mode == 1 ||
+                         (tu.nbr_mode == 2 && prefer_compact((double)g.n_fluid / (double)g.nn,
+                                                             segs_per_item(tu, g.nseg), lattice, prec));
     return g.device_bytes() + (compact ? g.compact_bytes() : 0) +
            (size_t)njobs * job_bytes(g, lattice, prec, lattice == 2);
 }
@@ -78,7 +79,7 @@ int64_t submit(fp_server *S, std::unique_ptr<Request> r, const uint8_t *solid)
         return -(int64_t)status_of_geometry_error(e);
     }
     const int njobs = r->nruns + (r->validate ? 1 : 0);
-    r->bytes = request_bytes(*r->geo, S->lattice, S->prec, njobs, S->batch->tuning().nbr_mode);
+    r->bytes = request_bytes(*r->geo, S->lattice, S->prec, njobs, S->batch->tuning());
     if((int64_t)r->bytes > S->budget) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->st.rejected++;
