@@ -89,7 +89,7 @@ def bench_steps(ctx, mask, prec, warmup, steps, tune=None, flux=0):
     t = None
     if tune is not None:
         t = _lib.Tuning(tune[0], tune[1], 200, tune[2], tune[3], tune[4],
-                        tune[5] if len(tune) > 5 else 0, tune[6] if len(tune) > 6 else 0)
+                        *(tune[k] if len(tune) > k else 0 for k in (5, 6, 7, 8)))
     st = ctx.lib.fp_bench_steps(ctx.handle, _lib.u8ptr(mask), nx, ny, nz, 3, prec,
                                 _lib.f64ptr(F), warmup, steps, C.byref(t) if t else None,
                                 int(flux), C.byref(out))
@@ -125,7 +125,8 @@ def apply_tuning(ctx, name, nbr_mode=None):
     if r:
         t = _lib.Tuning(r["local_cpt"], r["nbr_waves"], r["chunk_steps"], r["nontemporal"],
                         r["local_bpc"], r["nbr_bpc"],
-                        r.get("nbr_mode", 2) if nbr_mode is None else nbr_mode, r.get("nbr_segs", 0))
+                        r.get("nbr_mode", 2) if nbr_mode is None else nbr_mode, r.get("nbr_segs", 0),
+                        r.get("local_wgs", 0), r.get("nbr_wgs", 0))
         ctx.lib.fp_set_tuning(ctx.handle, C.byref(t))
 
 
@@ -418,12 +419,14 @@ def precision_study(ctx, seed, steps):
 
 
 def fixed_tuning(spec):
-    """launch shapes given as cpt,waves,nontemporal,local_bpc,nbr_bpc[,nbr_mode[,nbr_segs]]"""
+    """launch shapes given as
+    cpt,waves,nontemporal,local_bpc,nbr_bpc[,nbr_mode[,nbr_segs[,local_wgs[,nbr_wgs]]]]"""
     v = [int(x) for x in spec.split(",")]
     c, w, n, lb, nb = v[:5]
+    opt = [v[k] if len(v) > k else 0 for k in (5, 6, 7, 8)]
     return {"local_cpt": c, "nbr_waves": w, "nontemporal": n, "local_bpc": lb,
-            "nbr_bpc": nb, "nbr_mode": v[5] if len(v) > 5 else 0,
-            "nbr_segs": v[6] if len(v) > 6 else 0, "chunk_steps": 200, "fixed": True}
+            "nbr_bpc": nb, "nbr_mode": opt[0], "nbr_segs": opt[1], "local_wgs": opt[2],
+            "nbr_wgs": opt[3], "chunk_steps": 200, "fixed": True}
 
 
 def large_lattice(ctx, seed, steps, solve, tune=""):
@@ -449,7 +452,7 @@ def large_lattice(ctx, seed, steps, solve, tune=""):
            "step_kernel_GBs": o.n_fluid * 152 / (0.5 * (o.local_ms + o.nbr_ms) / 1e3) / 1e9,
            "local_ms": o.local_ms, "neighbour_ms": o.nbr_ms,
            "tuning": {k: TUNED["f32"].get(k, 0) for k in ("local_cpt", "nbr_waves", "nbr_bpc", "nbr_mode",
-                                                          "nbr_segs")}}
+                                                          "nbr_segs", "local_wgs", "nbr_wgs")}}
     log(f"512^3: {res['mlups']:.0f} MLUPS, {res['step_kernel_GBs']:.0f} GB/s")
     if solve:
         row = solve_logged(ctx, m, "f32", **SOLVE_CFG)
@@ -647,7 +650,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 rate")
     ap.add_argument("--no-small", action="store_true", help="skip the small-lattice step times")
     ap.add_argument("--tune", default="", help="fixed launch shapes "
-                    "'cpt,waves,nt,local_bpc,nbr_bpc[,nbr_mode[,nbr_segs]]' for both precisions instead of "
+                    "'cpt,waves,nt,local_bpc,nbr_bpc[,nbr_mode[,nbr_segs[,local_wgs[,nbr_wgs]]]]' for both "
+                    "precisions instead of "
                     "running the autotuner")
     ap.add_argument("--batched", type=int, default=48, help="structures of the batched set "
                     "solved batched in FP32 (0 = skip)")
@@ -720,7 +724,7 @@ def main():
 
     def shape(r):
         return (r["local_cpt"], r["nbr_waves"], r["nontemporal"], r["local_bpc"], r["nbr_bpc"],
-                r.get("nbr_mode", 0), r.get("nbr_segs", 0))
+                r.get("nbr_mode", 0), r.get("nbr_segs", 0), r.get("local_wgs", 0), r.get("nbr_wgs", 0))
     t32 = shape(tuned["f32"])
     log(f"FP32 D3Q19 {a.size}^3: warmup {warm}, timed {steps} steps")
     o32 = bench_steps(ctx, mask, 1, warm, steps, t32)

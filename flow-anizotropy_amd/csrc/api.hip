@@ -535,6 +535,7 @@ static void copy_tuning(const fp_tuning *t, Tuning &tu)
     tu.nontemporal = t->nontemporal; tu.local_bpc = t->local_bpc;
     tu.nbr_bpc = t->nbr_bpc; tu.nbr_mode = t->nbr_mode;
     tu.nbr_segs = t->nbr_segs > 0 ? t->nbr_segs : Tuning().nbr_segs;
+    tu.local_wgs = t->local_wgs; tu.nbr_wgs = t->nbr_wgs;
 }
 
 int fp_verify_batch_tables(const uint8_t *const *solids, const int32_t *dims, int n, int lattice,
@@ -962,7 +963,8 @@ static bool tuning_ok(const fp_tuning *t)
            t->local_bpc >= 0 && t->local_bpc <= 32 && t->nbr_bpc >= 0 && t->nbr_bpc <= 32 &&
            (t->nbr_mode >= 0 && t->nbr_mode <= 2) &&
            (t->nbr_segs == 0 || t->nbr_segs == 1 || t->nbr_segs == 2 || t->nbr_segs == 4 ||
-            t->nbr_segs == 8);
+            t->nbr_segs == 8) &&
+           t->local_wgs >= 0 && t->local_wgs <= 32 && t->nbr_wgs >= 0 && t->nbr_wgs <= 32;
 }
 
 extern "C" int fp_set_tuning(fp_ctx *ctx, const fp_tuning *t)
@@ -984,6 +986,8 @@ extern "C" void fp_get_tuning(fp_ctx *ctx, fp_tuning *t)
     t->nbr_bpc = ctx->tune.nbr_bpc;
     t->nbr_mode = ctx->tune.nbr_mode;
     t->nbr_segs = ctx->tune.nbr_segs;
+    t->local_wgs = ctx->tune.local_wgs;
+    t->nbr_wgs = ctx->tune.nbr_wgs;
 }
 
 extern "C" int fp_bench_steps(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
@@ -1089,25 +1093,32 @@ extern "C" int fp_bench_copy(fp_ctx *ctx, int64_t bytes, int iters, double *ms_p
 // neighbour shape i mod FP_TUNE_NNBR (the kernels are timed independently by
 // events).
 // ---------------------------------------------------------------------------
-// local step: cells/thread, blocks/CU (0: one per item), non-temporal
-static const int TUNE_LOCAL[24][3] = {
-    {1, 0, 0}, {1, 2, 0}, {1, 8, 0}, {1, 16, 0}, {2, 0, 0}, {2, 2, 0}, {2, 8, 0}, {2, 16, 0},
-    {4, 0, 0}, {4, 2, 0}, {4, 8, 0}, {4, 16, 0}, {1, 0, 1}, {1, 2, 1}, {1, 8, 1}, {1, 16, 1},
-    {2, 0, 1}, {2, 2, 1}, {2, 8, 1}, {2, 16, 1}, {4, 0, 1}, {4, 2, 1}, {4, 8, 1}, {4, 16, 1}};
+// local step: cells/thread, blocks/CU (0: one per item), non-temporal,
+// resident workgroups per CU (local_wgs, 0: no cap)
+static const int TUNE_LOCAL[FP_TUNE_NLOCAL][4] = {
+    {1, 0, 0, 0}, {2, 0, 0, 0}, {4, 0, 0, 0},
+    {1, 0, 1, 0}, {1, 2, 1, 0}, {1, 8, 1, 0}, {1, 16, 1, 0},
+    {2, 0, 1, 0}, {2, 2, 1, 0}, {2, 8, 1, 0}, {2, 16, 1, 0},
+    {4, 0, 1, 0}, {4, 2, 1, 0}, {4, 8, 1, 0}, {4, 16, 1, 0},
+    {1, 0, 1, 2}, {1, 0, 1, 3}, {1, 0, 1, 4}, {1, 0, 1, 5}, {1, 0, 1, 6},
+    {2, 0, 1, 2}, {2, 0, 1, 3}, {2, 0, 1, 4}, {4, 0, 1, 2}};
 // neighbour step: segments per workgroup (row-segment form), blocks/CU (0:
 // one per item), form (0 row segments, 1 compact), x-adjacent segments of a
-// row among them (nbr_segs)
-static const int TUNE_NBR[FP_TUNE_NNBR][4] = {
-    {4, 0, 0, 4}, {8, 0, 0, 4}, {8, 0, 0, 8}, {2, 0, 0, 2}, {4, 0, 0, 2}, {8, 0, 0, 2},
-    {2, 0, 0, 1}, {4, 0, 0, 1}, {8, 0, 0, 1}, {4, 8, 0, 4}, {4, 4, 0, 1},
-    {1, 0, 1, 1}, {1, 2, 1, 1}, {1, 4, 1, 1}, {1, 8, 1, 1}, {1, 16, 1, 1}};
+// row among them (nbr_segs), resident workgroups per CU (nbr_wgs)
+static const int TUNE_NBR[FP_TUNE_NNBR][5] = {
+    {4, 0, 0, 4, 0}, {8, 0, 0, 4, 0}, {8, 0, 0, 8, 0}, {2, 0, 0, 2, 0}, {4, 0, 0, 2, 0},
+    {8, 0, 0, 2, 0}, {2, 0, 0, 1, 0}, {4, 0, 0, 1, 0}, {8, 0, 0, 1, 0}, {4, 0, 0, 4, 5},
+    {4, 0, 0, 4, 6}, {1, 0, 1, 1, 0}, {1, 2, 1, 1, 0}, {1, 0, 1, 1, 5}, {1, 8, 1, 1, 0},
+    {1, 16, 1, 1, 0}};
 
 extern "C" void fp_tune_shape(int kind, int i, int *a, int *b, int *c)
 {
-    if(kind == 0 && i >= 0 && i < 24) { *a = TUNE_LOCAL[i][0]; *b = TUNE_LOCAL[i][1]; *c = TUNE_LOCAL[i][2]; }
+    *a = *b = *c = 0;
+    if(kind == 0 && i >= 0 && i < FP_TUNE_NLOCAL) { *a = TUNE_LOCAL[i][0]; *b = TUNE_LOCAL[i][1]; *c = TUNE_LOCAL[i][2]; }
     else if(kind == 1 && i >= 0 && i < FP_TUNE_NNBR) { *a = TUNE_NBR[i][0]; *b = TUNE_NBR[i][1]; *c = TUNE_NBR[i][2]; }
-    else if(kind == 2 && i >= 0 && i < FP_TUNE_NNBR) { *a = TUNE_NBR[i][3]; *b = *c = 0; }
-    else { *a = *b = *c = 0; }
+    else if(kind == 2 && i >= 0 && i < FP_TUNE_NNBR) *a = TUNE_NBR[i][3];
+    else if(kind == 3 && i >= 0 && i < FP_TUNE_NLOCAL) *a = TUNE_LOCAL[i][3];
+    else if(kind == 4 && i >= 0 && i < FP_TUNE_NNBR) *a = TUNE_NBR[i][4];
 }
 
 extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int lattice,
@@ -1147,8 +1158,9 @@ extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, in
         const int i = r % FP_TUNE_NLOCAL, n = r % FP_TUNE_NNBR;
         Tuning t = ctx->tune;
         t.local_cpt = TUNE_LOCAL[i][0]; t.local_bpc = TUNE_LOCAL[i][1]; t.nontemporal = TUNE_LOCAL[i][2];
+        t.local_wgs = TUNE_LOCAL[i][3];
         t.nbr_waves = TUNE_NBR[n][0]; t.nbr_bpc = TUNE_NBR[n][1]; t.nbr_mode = TUNE_NBR[n][2];
-        t.nbr_segs = TUNE_NBR[n][3];
+        t.nbr_segs = TUNE_NBR[n][3]; t.nbr_wgs = TUNE_NBR[n][4];
         b.set_tuning(t);
         Batch::KernelTimes kw, kt;
         e = b.time_steps(2, kw);
@@ -1172,6 +1184,8 @@ extern "C" int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, in
     best->nbr_bpc = TUNE_NBR[bn][1];
     best->nbr_mode = TUNE_NBR[bn][2];
     best->nbr_segs = TUNE_NBR[bn][3];
+    best->local_wgs = TUNE_LOCAL[bl][3];
+    best->nbr_wgs = TUNE_NBR[bn][4];
     best->chunk_steps = ctx->chunk;
     if(local_ms) for(int i = 0; i < FP_TUNE_NLOCAL; i++) local_ms[i] = lt[i];
     if(nbr_ms) for(int i = 0; i < FP_TUNE_NNBR; i++) nbr_ms[i] = nt[i];

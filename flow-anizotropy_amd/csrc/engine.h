@@ -189,6 +189,7 @@ private:
     InstState *d_state_ = nullptr;
     InstState *d_state2_ = nullptr;          // the other state array of the deferred finish
     bool defer_ = false;                     // the mode of the last chunk (or of enqueue_steps)
+    bool capped_ = false;                    // step launches apply Tuning::local_wgs / nbr_wgs
     bool defer_ok_ = false;                  // every instance within DEFER_MAX_PARTIALS
     bool defer_small_ = false;               // ... and within REDUCE_SPAN
     LocalWork *d_lwork_ = nullptr;

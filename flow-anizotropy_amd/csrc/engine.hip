@@ -1092,13 +1092,19 @@ void Batch::launch_step(bool local, const Defer *df)
         st = d.sout == d_state2_ ? d_state_ : d_state2_;
         d.slot0 = 0;
     }
+    // The workgroup caps pay where steps are mostly quiet (no flux sum): 256^3
+    // FP32, 0.2875 against 0.2958 ms a step.  With a flux sum on every step each
+    // work item ends in an LDS exchange and a barrier, which fewer resident
+    // workgroups hide worse (0.342 against 0.324 ms), so chunks without a finish
+    // plan (dense or deferred finish) launch uncapped.
+    Tuning tu = tune_;
+    if(!capped_) tu.local_wgs = tu.nbr_wgs = 0;
     if(local) {
-        kern_step(lattice_, prec_, true, tune_, ad_, d_inst_, st, d_lwork_, n_lwork_, d_lpart_, stream_,
+        kern_step(lattice_, prec_, true, tu, ad_, d_inst_, st, d_lwork_, n_lwork_, d_lpart_, stream_,
                   df ? &d : nullptr);
         return;
     }
     // the row-segment items, then the compact ones (their partial slots follow)
-    Tuning tu = tune_;
     if(n_nwork_ > 0) {
         tu.nbr_mode = 0;
         kern_step(lattice_, prec_, false, tu, ad_, d_inst_, st, d_nwork_, n_nwork_, d_npart_, stream_,
@@ -1114,6 +1120,7 @@ void Batch::launch_step(bool local, const Defer *df)
 
 void Batch::enqueue_steps(int n, const std::vector<int> *adv)
 {
+    capped_ = adv != nullptr;
     if(!defer_) {
         for(int t = 0; t < n; t++) {
             const bool local = (t & 1) == 0;     // every live job starts at an even step count
@@ -1149,6 +1156,12 @@ std::string Batch::time_steps(int steps, KernelTimes &kt)
     }
     if(n_inst_ == 0) return "nothing to step";
     steps &= ~1;
+    {
+        // the caps where a graph of these steps would apply them
+        std::vector<int> adv;
+        bool cd = false;
+        capped_ = choose_plan(0, steps, 1, adv, cd);
+    }
     const int nev = 4 * steps + 2;
     std::vector<hipEvent_t> ev(nev);
     for(auto &x : ev)

@@ -830,15 +830,56 @@ static int grid_for(int nwork, int blocks_per_cu)
     return (int)std::max<int64_t>(std::min<int64_t>(nwork, g), std::min(nwork, 8));
 }
 
+// Dynamic LDS that holds a kernel to wgs resident workgroups per CU (0: no cap).
+// The kernel never touches it: its only effect is that no more workgroups fit in
+// a CU's LDS.  The footprint (the kernel's static LDS plus this) is a multiple of
+// 5120 bytes, so allocation granules of 1024 or 1280 bytes do not round it up,
+// and it lies in (lds/(wgs+1), lds/wgs] for the CU's lds bytes.  A cap that no
+// such footprint reaches (wgs 1 or 7 and up with 160 KiB, or above what one
+// workgroup may allocate) returns 0 and is not applied.
+static size_t lds_for_wgs(const void *kern, int wgs)
+{
+    if(wgs <= 0) return 0;
+    static int lds_cu[64] = {0}, lds_wg[64] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if(dev < 0 || dev >= 64) return 0;
+    if(!lds_cu[dev]) {
+        int cu = 0, wg = 0;
+        if(hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+           hipDeviceGetAttribute(&wg, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+           cu <= 0 || wg <= 0)
+            return 0;
+        lds_wg[dev] = wg;
+        lds_cu[dev] = cu;
+    }
+    hipFuncAttributes a;
+    if(hipFuncGetAttributes(&a, kern) != hipSuccess) return 0;
+    const size_t G = 5120, cu = (size_t)lds_cu[dev];
+    const size_t F = std::min(cu / (size_t)wgs, (size_t)lds_wg[dev]) / G * G;
+    if(F <= cu / (size_t)(wgs + 1) || F <= a.sharedSizeBytes) return 0;
+    return F - a.sharedSizeBytes;
+}
+
 template<class L, int P, int CPT, bool NT>
 static void launch_local(const Tuning &tu, bool wide, const InstDesc *insts, const InstState *st,
                          const LocalWork *lw, int nwork, Partial *part, const Defer &df, hipStream_t s)
 {
     const dim3 g(grid_for(nwork, tu.local_bpc)), b(256);
     // the deferred finish runs on small lattices only (32-bit indices)
-    if(df.cfg) hipLaunchKernelGGL((k_local<L, P, CPT, NT, false, true>), g, b, 0, s, insts, st, lw, nwork, part, df);
-    else if(wide) hipLaunchKernelGGL((k_local<L, P, CPT, NT, true, false>), g, b, 0, s, insts, st, lw, nwork, part, df);
-    else     hipLaunchKernelGGL((k_local<L, P, CPT, NT, false, false>), g, b, 0, s, insts, st, lw, nwork, part, df);
+    if(df.cfg) {
+        const void *k = (const void *)&k_local<L, P, CPT, NT, false, true>;
+        hipLaunchKernelGGL((k_local<L, P, CPT, NT, false, true>), g, b, lds_for_wgs(k, tu.local_wgs), s,
+                           insts, st, lw, nwork, part, df);
+    } else if(wide) {
+        const void *k = (const void *)&k_local<L, P, CPT, NT, true, false>;
+        hipLaunchKernelGGL((k_local<L, P, CPT, NT, true, false>), g, b, lds_for_wgs(k, tu.local_wgs), s,
+                           insts, st, lw, nwork, part, df);
+    } else {
+        const void *k = (const void *)&k_local<L, P, CPT, NT, false, false>;
+        hipLaunchKernelGGL((k_local<L, P, CPT, NT, false, false>), g, b, lds_for_wgs(k, tu.local_wgs), s,
+                           insts, st, lw, nwork, part, df);
+    }
 }
 
 template<class L, int P, int AD, bool DF>
@@ -849,10 +890,12 @@ static void launch_nbr(const Tuning &tu, const InstDesc *insts, const InstState 
     // it never uses non-temporal accesses (measured 1.6-1.8x the bytes)
     if(tu.nbr_mode == 1) {
         const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(256);
-        hipLaunchKernelGGL((k_nbr_c<L, P, AD, DF>), g, b, 0, s, insts, st, (const LocalWork *)work, nwork, part, df);
+        const size_t lds = lds_for_wgs((const void *)&k_nbr_c<L, P, AD, DF>, tu.nbr_wgs);
+        hipLaunchKernelGGL((k_nbr_c<L, P, AD, DF>), g, b, lds, s, insts, st, (const LocalWork *)work, nwork, part, df);
     } else {
         const dim3 g(grid_for(nwork, tu.nbr_bpc)), b(64 * tu.nbr_waves);
-        hipLaunchKernelGGL((k_nbr<L, P, AD, DF>), g, b, 0, s, insts, st, (const NbrWork *)work, nwork, part, df);
+        const size_t lds = lds_for_wgs((const void *)&k_nbr<L, P, AD, DF>, tu.nbr_wgs);
+        hipLaunchKernelGGL((k_nbr<L, P, AD, DF>), g, b, lds, s, insts, st, (const NbrWork *)work, nwork, part, df);
     }
 }
 

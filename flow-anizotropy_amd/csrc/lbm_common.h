@@ -631,7 +631,7 @@ struct Defer {
 // CU looping over items: local step 0.253 against 0.267 ms (FP32), 0.533
 // against 0.572 (FP64).
 struct Tuning {
-    int local_cpt = 2;      // cells per thread in the local kernel: 1, 2 or 4
+    int local_cpt = 1;      // cells per thread in the local kernel: 1, 2 or 4
     int nbr_waves = 4;      // wavefronts per workgroup in the neighbour kernel: 1..8, one
                             // row segment each
     int nbr_segs = 4;       // of those, x-adjacent segments of one row (1, 2, 4 or 8, at
@@ -645,6 +645,10 @@ struct Tuning {
     int nbr_mode = 2;       // neighbour step: 0 = one wavefront per row segment (link
                             // records), 1 = 64 fluid cells per wavefront (compact table),
                             // 2 = per instance by porosity (COMPACT_BELOW_POROSITY)
+    int local_wgs = 3;      // resident workgroups per CU the local kernel is held to (LDS
+                            // it does not use, sized so that no more fit) in chunks that
+                            // run a finish plan; 0: as many as its registers allow
+    int nbr_wgs = 5;        // the same for the neighbour kernels
 };
 
 // Below this porosity the compact neighbour step is the faster form: it keeps

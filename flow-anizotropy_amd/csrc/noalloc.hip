@@ -13,6 +13,7 @@
 // stream; the results are bit-identical to fp_solve_dir3d's.
 #include "solve_common.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -182,18 +183,22 @@ extern "C" int fp_solve_dir3d_noalloc(fp_ctx *ctx, const uint8_t *solid, int nx,
     std::vector<int> adv(chunk, 1);
     while(S.phase < PH_DONE) {
         if(sparse) plan_finishes(&j.cfg, &S, 1, 0, chunk, 1, adv);
+        // the workgroup caps only in chunks that skip finishes, as a batch
+        // applies them (a flux sum every step runs faster uncapped)
+        Tuning tc = tu;
+        if(!sparse || std::count(adv.begin(), adv.end(), 0) == 0) tc.local_wgs = tc.nbr_wgs = 0;
         // every chunk starts at an even step count: local (odd) step first
         for(int step = 0; step < chunk; step++) {
             const bool local = (step & 1) == 0;
             if(local) {
-                kern_step(3, prec, true, tu, AD_B32, d_inst, d_state, base + L.lwork, n_l,
+                kern_step(3, prec, true, tc, AD_B32, d_inst, d_state, base + L.lwork, n_l,
                           (Partial *)(base + L.lpart), s);
                 if(adv[step] > 0)
                     kern_finish(n_lr, (const RedWork *)(base + L.lredw), d_inst, d_cfg, d_state,
                                 (const Partial *)(base + L.lpart), (RPartial *)(base + L.lred),
                                 (unsigned int *)(base + L.count), true, s, adv[step]);
             } else {
-                kern_step(3, prec, false, tu, AD_B32, d_inst, d_state, base + L.nwork, n_n,
+                kern_step(3, prec, false, tc, AD_B32, d_inst, d_state, base + L.nwork, n_n,
                           (Partial *)(base + L.npart), s);
                 if(adv[step] > 0)
                     kern_finish(n_nr, (const RedWork *)(base + L.nredw), d_inst, d_cfg, d_state,

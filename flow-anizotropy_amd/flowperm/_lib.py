@@ -98,7 +98,8 @@ class BenchOut(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("local_cpt", C.c_int32), ("nbr_waves", C.c_int32), ("chunk_steps", C.c_int32),
                 ("nontemporal", C.c_int32), ("local_bpc", C.c_int32), ("nbr_bpc", C.c_int32),
-                ("nbr_mode", C.c_int32), ("nbr_segs", C.c_int32)]
+                ("nbr_mode", C.c_int32), ("nbr_segs", C.c_int32), ("local_wgs", C.c_int32),
+                ("nbr_wgs", C.c_int32)]
 
 
 class ServerConfig(C.Structure):

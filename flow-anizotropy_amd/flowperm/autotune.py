@@ -21,28 +21,39 @@ from . import _lib
 
 
 FIELDS = ("local_cpt", "nbr_waves", "chunk_steps", "nontemporal", "local_bpc", "nbr_bpc",
-          "nbr_mode", "nbr_segs")
+          "nbr_mode", "nbr_segs", "local_wgs", "nbr_wgs")
 N_LOCAL, N_NBR = 24, 16              # FP_TUNE_NLOCAL, FP_TUNE_NNBR
 
 
 def shapes(lib):
-    """The shapes fp_autotune tries: (local list of (cpt, bpc, nt), neighbour
-    list of (waves, bpc, mode, segs))."""
+    """The shapes fp_autotune tries: (local list of (cpt, bpc, nt, wgs),
+    neighbour list of (waves, bpc, mode, segs, wgs))."""
     a, b, c = C.c_int(), C.c_int(), C.c_int()
     loc, nbr = [], []
     for i in range(N_LOCAL):
         lib.fp_tune_shape(0, i, C.byref(a), C.byref(b), C.byref(c))
-        loc.append((a.value, b.value, c.value))
+        cpt, bb, nt = a.value, b.value, c.value
+        lib.fp_tune_shape(3, i, C.byref(a), C.byref(b), C.byref(c))
+        loc.append((cpt, bb, nt, a.value))
     for i in range(N_NBR):
         lib.fp_tune_shape(1, i, C.byref(a), C.byref(b), C.byref(c))
         w, bb, m = a.value, b.value, c.value
         lib.fp_tune_shape(2, i, C.byref(a), C.byref(b), C.byref(c))
-        nbr.append((w, bb, m, a.value))
+        sg = a.value
+        lib.fp_tune_shape(4, i, C.byref(a), C.byref(b), C.byref(c))
+        nbr.append((w, bb, m, sg, a.value))
     return loc, nbr
 
 
-def nbr_name(w, b, m, s=1):
-    return f"compact_bpc{b}" if m == 1 else f"waves{w}_segs{s}_bpc{b}"
+def nbr_name(w, b, m, s=1, g=0):
+    """waves4_segs4_bpc0, compact_bpc0; _wgs5 when held to 5 workgroups per CU"""
+    cap = f"_wgs{g}" if g else ""
+    return (f"compact_bpc{b}" if m == 1 else f"waves{w}_segs{s}_bpc{b}") + cap
+
+
+def local_name(c, b, n, g=0):
+    """cpt2_bpc0_nt1; _wgs3 when held to 3 workgroups per CU"""
+    return f"cpt{c}_bpc{b}_nt{n}" + (f"_wgs{g}" if g else "")
 
 
 def cache_path() -> Path:
@@ -67,8 +78,9 @@ def device_name() -> str:
 def _key(lattice, precision, n_cells, porosity):
     # the best neighbour-step form depends on how sparse the lattice is
     size_class = "small" if n_cells < 2_000_000 else "large"
-    # (the neighbour shapes gained nbr_segs: entries of older libraries lack it)
-    return f"{device_name()}/{lattice}/{precision}/{size_class}/phi{round(porosity, 1):.1f}/s"
+    # (the shapes gained nbr_segs, then the workgroup caps: entries of older
+    # libraries lack them)
+    return f"{device_name()}/{lattice}/{precision}/{size_class}/phi{round(porosity, 1):.1f}/w"
 
 
 def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
@@ -99,9 +111,8 @@ def tune(ctx, mask, lattice=3, precision=1, steps=40, use_cache=True):
         raise _lib.FlowpermError(st, ctx.last_error())
     rec = {k: getattr(best, k) for k in FIELDS}
     loc, nbr = shapes(ctx.lib)
-    rec["local_ms"] = {f"cpt{c}_bpc{b}_nt{n}": round(v, 5) for (c, b, n), v in zip(loc, lms)}
-    rec["nbr_ms"] = {nbr_name(w, b, m, sg): round(v, 5) for (w, b, m, sg), v in zip(nbr, nms)
-                     if v < 1e29}
+    rec["local_ms"] = {local_name(*sh): round(v, 5) for sh, v in zip(loc, lms)}
+    rec["nbr_ms"] = {nbr_name(*sh): round(v, 5) for sh, v in zip(nbr, nms) if v < 1e29}
     if use_cache:
         cache[key] = rec
         try:

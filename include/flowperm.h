@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 9
+#define FP_ABI_VERSION 10
 
 enum fp_status {
     FP_OK              = 0,   /* all directions converged            (exit 0) */
@@ -321,9 +321,9 @@ typedef struct fp_bench_out {
 } fp_bench_out;
 
 /* Launch shapes of the step kernels; the autotuner picks them per device.  A new
-   context starts with local_cpt 2, nbr_waves 4, chunk_steps 200, nontemporal 1,
-   local_bpc 0, nbr_bpc 0, nbr_mode 2, nbr_segs 4 (the shapes tuned for D3Q19 at
-   256^3). */
+   context starts with local_cpt 1, nbr_waves 4, chunk_steps 200, nontemporal 1,
+   local_bpc 0, nbr_bpc 0, nbr_mode 2, nbr_segs 4, local_wgs 3, nbr_wgs 5 (the
+   shapes tuned for D3Q19 FP32 at 256^3). */
 typedef struct fp_tuning {
     int32_t local_cpt;        /* cells per thread, local kernel: 1, 2 or 4            */
     int32_t nbr_waves;        /* row segments (wavefronts) per workgroup, neighbour
@@ -342,6 +342,17 @@ typedef struct fp_tuning {
                                  and lowered to a divisor of the row's 64-cell
                                  segments; the others stack rows along z, along y in
                                  2D); 0 = 4 (default)                                 */
+    int32_t local_wgs;        /* resident workgroups per CU the local kernel is held
+                                 to, through LDS it does not use: 1..6 on a 160 KiB
+                                 CU (a cap no LDS size reaches is not applied); 0 =
+                                 as many as its registers allow.  Applied in chunks
+                                 that run a finish plan (most steps without a flux
+                                 sum); chunks with a finish after every step, or the
+                                 deferred finish, launch uncapped.  Fewer streams in
+                                 flight move more bytes: 256^3 FP32 local step 0.235
+                                 against 0.245 ms at 3.                              */
+    int32_t nbr_wgs;          /* the same for the neighbour kernels (0.348 against
+                                 0.353 ms at 5)                                      */
 } fp_tuning;
 
 /* Set / read the launch shapes a context uses for its solves and batches. */
@@ -398,7 +409,8 @@ int fp_autotune(fp_ctx *ctx, const uint8_t *solid, int nx, int ny, int nz, int l
                 int precision, int steps, fp_tuning *best, double *local_ms, double *nbr_ms);
 /* The i-th shape tried by fp_autotune: kind 0 = local (cpt, bpc, nontemporal in
  * a, b, c), kind 1 = neighbour (waves, bpc, nbr_mode in a, b, c), kind 2 = the
- * same neighbour shape's nbr_segs (in a). */
+ * same neighbour shape's nbr_segs (in a), kind 3 = the local shape's local_wgs,
+ * kind 4 = the neighbour shape's nbr_wgs (in a). */
 void fp_tune_shape(int kind, int i, int *a, int *b, int *c);
 
 /* ------------------------------------------------------------------------ */
