@@ -1096,9 +1096,11 @@ void Batch::launch_step(bool local, const Defer *df)
     // FP32, 0.2875 against 0.2958 ms a step.  With a flux sum on every step each
     // work item ends in an LDS exchange and a barrier, which fewer resident
     // workgroups hide worse (0.342 against 0.324 ms), so chunks without a finish
-    // plan (dense or deferred finish) launch uncapped.
+    // plan (dense or deferred finish) launch uncapped.  D2Q9 keeps 9 populations
+    // of a cell in flight, not 19, and runs slower capped (1024^2 FP32: 9.3
+    // against 8.4 us a step): the caps are for D3Q19 only.
     Tuning tu = tune_;
-    if(!capped_) tu.local_wgs = tu.nbr_wgs = 0;
+    if(!capped_ || lattice_ != 3) tu.local_wgs = tu.nbr_wgs = 0;
     if(local) {
         kern_step(lattice_, prec_, true, tu, ad_, d_inst_, st, d_lwork_, n_lwork_, d_lpart_, stream_,
                   df ? &d : nullptr);

@@ -646,8 +646,8 @@ struct Tuning {
                             // records), 1 = 64 fluid cells per wavefront (compact table),
                             // 2 = per instance by porosity (COMPACT_BELOW_POROSITY)
     int local_wgs = 3;      // resident workgroups per CU the local kernel is held to (LDS
-                            // it does not use, sized so that no more fit) in chunks that
-                            // run a finish plan; 0: as many as its registers allow
+                            // it does not use, sized so that no more fit) in D3Q19 chunks
+                            // that run a finish plan; 0: as many as its registers allow
     int nbr_wgs = 5;        // the same for the neighbour kernels
 };
 

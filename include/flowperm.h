@@ -345,10 +345,10 @@ typedef struct fp_tuning {
     int32_t local_wgs;        /* resident workgroups per CU the local kernel is held
                                  to, through LDS it does not use: 1..6 on a 160 KiB
                                  CU (a cap no LDS size reaches is not applied); 0 =
-                                 as many as its registers allow.  Applied in chunks
-                                 that run a finish plan (most steps without a flux
-                                 sum); chunks with a finish after every step, or the
-                                 deferred finish, launch uncapped.  Fewer streams in
+                                 as many as its registers allow.  Applied to D3Q19
+                                 chunks that run a finish plan (most steps without a
+                                 flux sum); chunks with a finish after every step or
+                                 the deferred finish, and D2Q9, launch uncapped.  Fewer streams in
                                  flight move more bytes: 256^3 FP32 local step 0.235
                                  against 0.245 ms at 3.                              */
     int32_t nbr_wgs;          /* the same for the neighbour kernels (0.348 against
